@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Throughput of the coupled two-phase LBM step on one MI355X.
+
+Workload (default): 3D rising bubble, D3Q19 for both distributions, 256^3
+interior nodes (N = 258 with CLIP's in-N ghost ring), fully periodic, sphere
+R = 32, rhoRatio 1000, muRatio 100, Eotvos 10, Morton 1e-3, interface width 4,
+synthetic initial state from the case definition (no input files).  One step
+= NSAllen::solve on every interior node (ghost refresh + phase sum + fused
+collide/stream kernel).
+
+Prints one JSON line: MLUPS (whole job), the roofline of the dominant kernel
+(k_fused, HIP events on the solver's stream) and a CPU baseline (the FP64
+OpenMP oracle on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def pop_bytes(precision):
+    return {"f64": 8, "f32": 4, "mixed_f16": 2, "mixed_bf16": 2}[precision]
+
+
+def mac_bytes(precision):
+    return 8 if precision == "f64" else 4
+
+
+def workload_text(name):
+    from tests import cases
+    if name == "bubble3d256":
+        return cases.bubble3d(n=256, radius=32.0)
+    if name == "bubble3d128":
+        return cases.bubble3d(n=128, radius=16.0)
+    if name == "rti2d":
+        return cases.rti2d(256, 1024)
+    if name == "drop2d128":
+        return cases.drop2d(n=128, radius=25)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def load_traffic(kernel, workload, precision):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 PMC runs and
+    committed under profiles/ (None when no measurement exists)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(text, steps_cap, seconds_budget):
+    """FP64 OpenMP oracle (CLIP's per-node phases in CLIP's AoS layout) on the
+    same configuration; times up to `steps_cap` steps after initialisation."""
+    from clip_amd import Config
+    from tests.oracle_lib import Oracle
+
+    cfg = Config.from_string(text)
+    orc = Oracle(cfg).init()
+    interior = 1
+    for a in range(cfg.dim):
+        interior *= cfg.extents()[a] - 2
+    done, t0 = 0, time.perf_counter()
+    while done < steps_cap:
+        orc.step(1)
+        done += 1
+        if time.perf_counter() - t0 > seconds_budget:
+            break
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    del orc
+    return {"value": interior * done / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"{done} step(s) of the same workload after initialisation, FP64 oracle, "
+                      f"OpenMP over {cores} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="bubble3d256")
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "mixed_f16", "mixed_bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the solver runs on a single MI355X; "
+              "multi-GPU slab decomposition is a design note, not a run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from clip_amd import Config, Solver
+
+    torch.cuda.set_device(0)
+    text = workload_text(args.workload)
+    cfg = Config.from_string(text)
+    sol = Solver(cfg, args.precision).init_fresh()
+    inf = sol.info()
+    interior = int(inf.interior)
+    Q, D = inf.q, inf.dim
+
+    sol.step(args.warmup)
+    sol.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol.step(args.steps)
+    sol.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    mlups = interior * args.steps / (t1 - t0) / 1e6
+
+    # Per-kernel durations with HIP events on the solver's stream.
+    tsteps = min(args.steps, 50)
+    sol.enable_timing(True)
+    sol.step(tsteps)
+    tm = sol.last_timing()
+    sol.enable_timing(False)
+    fused_ms = tm["fused_ms"] / tsteps
+    sp, sm = pop_bytes(args.precision), mac_bytes(args.precision)
+    fused_bytes = interior * (4 * Q * sp + (2 * D + 1) * sm)  # f,g read+write; phi read; u read+write
+    achieved = fused_bytes / (fused_ms * 1e-3) / 1e9
+    step_bytes = 4 * Q * sp + (2 * D + 2) * sm
+    health = sol.check_health()
+    sol.close()
+    del sol
+
+    line = {
+        "metric": "MLUPS + % HBM peak (D3Q19 coupled step) FP64 vs mixed; adjoint/forward time",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision.replace("mixed_", "") if args.precision.startswith("mixed") else args.precision,
+        "data": "synthetic",
+        "config": {"workload": args.workload, "interior_nodes": interior, "lattice": f"D{D}Q{Q}",
+                   "precision": args.precision, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic("k_fused", args.workload, args.precision),
+                     "kernel": "k_fused", "kernel_ms": round(fused_ms, 4),
+                     "alg_bytes_per_launch": fused_bytes},
+        "step_hbm_frac": round(mlups * 1e6 * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms_per_step": {k: round(v / tsteps, 4) for k, v in tm.items()},
+        "phi_sum": health,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(text, args.cpu_steps, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,201 @@
+"""clip_amd — MI355X-native coupled two-phase LBM step (Allen–Cahn + WMRT).
+
+Python host layer over libclip_amd.so (the C-ABI of include/clip_amd.h).  It
+mirrors the reference's host surface:
+
+    InputData  (include/input/InputData.cuh)  -> Config.from_file("config.txt")
+    NSAllen    (include/equation/NsAllen.cuh) -> Solver: initialCondition +
+               deviceInitializer = init_fresh(), solve() = step(1),
+               macroscopic fields through get()
+    Reporter::sum (src/utils/Reporter.cu:95-114) -> Solver.sum("phi")
+    CheckPointer::save / load (src/io/CheckPointer.cu:53-102)
+                                              -> Solver.checkpoint_save / load
+
+All compute runs in the HIP library; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import ClipError, CASE, SIDES, BC, GEOM, PRECISION, FIELD
+
+__all__ = ["Config", "Solver", "InputData", "NSAllen", "ClipError", "load_library"]
+
+
+def load_library():
+    return _abi.load()
+
+
+class Config:
+    """CLIP configuration (SimParams + boundary map + geometry pool)."""
+
+    def __init__(self, raw=None):
+        self.raw = raw if raw is not None else _abi.ClipConfig()
+
+    @classmethod
+    def from_file(cls, path, dim=0):
+        c = _abi.ClipConfig()
+        _abi.check(_abi.load().clip_config_from_file(str(path).encode(), int(dim), ctypes.byref(c)))
+        return cls(c)
+
+    @classmethod
+    def from_string(cls, text, dim=0):
+        c = _abi.ClipConfig()
+        _abi.check(_abi.load().clip_config_from_string(text.encode(), int(dim), ctypes.byref(c)))
+        return cls(c)
+
+    def derive(self):
+        _abi.check(_abi.load().clip_config_derive(ctypes.byref(self.raw)))
+        return self
+
+    def copy(self):
+        c = _abi.ClipConfig()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self.raw), ctypes.sizeof(c))
+        return Config(c)
+
+    @property
+    def dim(self):
+        return self.raw.dim
+
+    @property
+    def q(self):
+        return 9 if self.raw.dim == 2 else 19
+
+    def extents(self):
+        N = list(self.raw.N)
+        if self.raw.dim == 2:
+            return (int(N[0]) + 2, int(N[1]) + 2, 1)
+        return (int(N[0]), int(N[1]), int(N[2]))
+
+    def interior_slices(self):
+        ex = self.extents()
+        if self.raw.dim == 2:
+            return (slice(1, ex[0] - 1), slice(1, ex[1] - 1), slice(0, 1))
+        return tuple(slice(1, e - 1) for e in ex)
+
+    def __getattr__(self, name):
+        if name == "raw":
+            raise AttributeError(name)
+        return getattr(self.raw, name)
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            cur = getattr(self.raw, k)
+            if isinstance(cur, ctypes.Array):
+                for i, x in enumerate(v):
+                    cur[i] = x
+            else:
+                setattr(self.raw, k, v)
+        return self
+
+
+InputData = Config
+
+
+class Solver:
+    """One solver handle (device buffers for one configuration)."""
+
+    def __init__(self, config, precision="f64"):
+        self.config = config
+        self.precision = precision
+        self._lib = _abi.load()
+        h = ctypes.c_void_p()
+        _abi.check(self._lib.clip_create(ctypes.byref(config.raw), PRECISION[precision], ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.clip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # NSAllen::initialCondition + deviceInitializer
+    def init_fresh(self):
+        _abi.check(self._lib.clip_init_fresh(self._h))
+        return self
+
+    # nsteps x NSAllen::solve
+    def step(self, nsteps=1):
+        _abi.check(self._lib.clip_step(self._h, int(nsteps)))
+        return self
+
+    solve = step
+
+    def synchronize(self):
+        _abi.check(self._lib.clip_synchronize(self._h))
+
+    def set_stream(self, stream_ptr):
+        _abi.check(self._lib.clip_set_stream(self._h, ctypes.c_void_p(int(stream_ptr))))
+
+    def info(self):
+        inf = _abi.ClipInfo()
+        _abi.check(self._lib.clip_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def ndof(self, field):
+        d = self.config.dim
+        return {"phi": 1, "rho": 1, "p": 1, "mu": 1, "u": d, "dphi": d, "normal": d}.get(field, self.config.q)
+
+    def get(self, field, layout="grid"):
+        """Field as float64.  layout="grid": array (ext_x, ext_y, ext_z, ndof)
+        (CLIP's AoS order); "aos": flat CLIP AoS; "soa": flat dof-major."""
+        ex = self.config.extents()
+        nd = self.ndof(field)
+        out = np.empty(ex[0] * ex[1] * ex[2] * nd, dtype=np.float64)
+        lay = 1 if layout == "soa" else 0
+        _abi.check(self._lib.clip_get_field(self._h, FIELD[field], lay,
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        if layout == "grid":
+            return out.reshape(ex[0], ex[1], ex[2], nd)
+        return out
+
+    def set(self, field, values, layout="grid"):
+        arr = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        lay = 1 if layout == "soa" else 0
+        _abi.check(self._lib.clip_set_field(self._h, FIELD[field], lay,
+                                            arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def interior(self, field):
+        a = self.get(field)
+        return a[self.config.interior_slices()]
+
+    def sum(self, field="phi"):
+        out = ctypes.c_double()
+        _abi.check(self._lib.clip_sum(self._h, FIELD[field], ctypes.byref(out)))
+        return out.value
+
+    def check_health(self):
+        """Reporter::print's divergence check; raises ClipError(ERR_DIVERGED)."""
+        out = ctypes.c_double()
+        _abi.check(self._lib.clip_check_health(self._h, ctypes.byref(out)))
+        return out.value
+
+    def checkpoint_save(self, directory):
+        _abi.check(self._lib.clip_checkpoint_save(self._h, str(directory).encode()))
+
+    def checkpoint_load(self, directory):
+        _abi.check(self._lib.clip_checkpoint_load(self._h, str(directory).encode()))
+
+    def enable_timing(self, on=True):
+        _abi.check(self._lib.clip_enable_timing(self._h, 1 if on else 0))
+
+    def last_timing(self):
+        vals = [ctypes.c_double() for _ in range(4)]
+        _abi.check(self._lib.clip_last_step_timing(self._h, *[ctypes.byref(v) for v in vals]))
+        return {"total_ms": vals[0].value, "ghost_ms": vals[1].value,
+                "phase_ms": vals[2].value, "fused_ms": vals[3].value}
+
+
+NSAllen = Solver

@@ -1,0 +1,924 @@
+// kernels.hpp — HIP device code of the coupled Allen–Cahn / WMRT step (gfx950).
+//
+// Layout in HBM: structure of arrays, x fastest.  Node (i, j, k) of the
+// ghost-inclusive grid has index n = i + j*ext_x + k*ext_x*ext_y, population q
+// of buffer A sits at A[q*nodes + n].  f and g live in two ping-pong buffers
+// each (post-collision values, ghost cells included); phi has one buffer with
+// its ghost ring; u has two (old / new).
+//
+// One time step is three launches:
+//   k_ghost_*   refresh the phi ghost ring from phi^n (periodic wrap, inlet,
+//               mirror) — the lagged ghost values the reference's stencils see
+//   k_phase     phi^{n+1} = sum_q g_post, g_post pulled from g* with every
+//               boundary fix-up resolved at pull time
+//   k_fused     pull f_post/g_post, phi stencil (mu, grad phi, normal),
+//               macroscopic p/u (from u_old), then both collisions; writes
+//               f*, g* into the other buffer and u into the other u buffer.
+// Reference phases folded into these kernels (NSAllen::solve, NsAllen.cu:881-892):
+// normal_FD, kernelCollisionMRTg/f, kernelPeriodicBoundary, JetBoundary,
+// kernelStreaming, kernelHalfBounceBack, kernelFreeConvect, kernelNeumann,
+// kernelMirrorBoundary, kernelMacroscopicg, Chemical_Potential,
+// Isotropic_Gradient, kernelMacroscopicf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include "lattice.hpp"
+
+namespace clipamd {
+
+enum : int {
+    BC_WALL = CLIP_BC_WALL,
+    BC_SLIP_WALL = CLIP_BC_SLIP_WALL,
+    BC_FREE_CONVECT = CLIP_BC_FREE_CONVECT,
+    BC_PERIODIC = CLIP_BC_PERIODIC,
+    BC_NEUMANN = CLIP_BC_NEUMANN,
+    BC_VELOCITY = CLIP_BC_VELOCITY,
+    BC_DO_NOTHING = CLIP_BC_DO_NOTHING
+};
+
+struct DevGrid {
+    int ext[3];
+    int dmin[3], dmax[3];
+    long long sy, sz;  // strides of j and k (i has stride 1)
+    long long nodes;
+    __host__ __device__ long long idx(int i, int j, int k) const { return i + j * sy + k * sz; }
+};
+
+struct DevBC {
+    int type[6];
+    int periodic[3];
+    int is_wall, is_fc, is_neumann, is_velocity, mirror_gate;
+    const unsigned char* vflag[6];  // per velocity side: 1 = ghost node inside the inlet disc
+};
+
+// Index of a ghost-plane (or boundary-plane) cell of side `side` within its
+// plane: the two tangential coordinates, lower axis fastest.
+__host__ __device__ inline long long plane_index(const DevGrid& g, int side, const int c[3]) {
+    const int a = side >> 1;
+    const int t1 = (a == 0) ? 1 : 0;
+    const int t2 = (a == 2) ? 1 : 2;
+    return (long long)c[t2] * g.ext[t1] + c[t1];
+}
+__host__ __device__ inline long long plane_size(const DevGrid& g, int side) {
+    const int a = side >> 1;
+    const int t1 = (a == 0) ? 1 : 0;
+    const int t2 = (a == 2) ? 1 : 2;
+    return (long long)g.ext[t1] * g.ext[t2];
+}
+
+template <class CT>
+struct Phys {
+    CT rho_h, rho_l, tau_h, tau_l, beta, kappa, mobility, W;
+    CT grav[3];
+    int case_type;
+};
+
+// ------------------------------------------------------------ storage
+// Plain conversion between storage and arithmetic types.  The 16-bit
+// policies store populations as scaled deviations from w_q * ref; `ref`
+// is 0 for f and phi-at-rest for g (see StoreMixed).
+template <class ST, class CT>
+struct StorePlain {
+    __device__ static CT ld(const ST* p, long long i, int /*q*/) { return (CT)p[i]; }
+    __device__ static void st(ST* p, long long i, int /*q*/, CT v) { p[i] = (ST)v; }
+};
+
+// ------------------------------------------------------------ arguments
+template <class ST, class CT>
+struct StepArgs {
+    DevGrid g;
+    DevBC bc;
+    Phys<CT> ph;
+    const ST* fin;   // f* of the buffer being pulled
+    const ST* gin;
+    ST* fout;        // f* written by the collision
+    ST* gout;
+    CT* phi;         // phi with ghost ring
+    const CT* uin;   // u of the previous macroscopic step (D planes); outflow fix-ups read it
+    CT* uout;        // u of this macroscopic step
+    const CT* ucol;  // u the collide-only mode collides with
+    CT* p;           // pressure (read by the collide-only mode; written by exports)
+    const CT* prev_f_in[6];  // outflow planes: f_post of the previous step
+    CT* prev_f_out[6];
+    const CT* prev_g_in[6];
+    CT* prev_g_out[6];
+};
+
+// ------------------------------------------------------------ pull with fix-ups
+// Post-streaming population q at interior node c, reproducing the reference's
+// streaming + boundary sequence:
+//   raw   kernelStreaming from the ghost-updated f (periodic wrap, inlet)
+//   lvl1  kernelHalfBounceBack (wall / slip wall, when any wall exists)
+//   lvl2  kernelFreeConvect (reads lvl1 of the inner neighbour, Jacobi)
+//   lvl3  kernelNeumann     (reads lvl2 of the inner neighbour)
+template <class L, class ST, class CT, class SP>
+struct Puller {
+    const ST* A;
+    const DevGrid& g;
+    const DevBC& bc;
+    const CT* uold;
+    const CT* const* prev;
+
+    __device__ CT raw(const int c[3], int q) const {
+        int s[3] = {c[0] - L::ex(q), c[1] - L::ey(q), c[2] - L::ez(q)};
+#pragma unroll
+        for (int a = 0; a < L::D; ++a) {
+            if (bc.periodic[a]) {
+                if (s[a] < g.dmin[a]) s[a] = g.dmax[a];
+                else if (s[a] > g.dmax[a]) s[a] = g.dmin[a];
+            }
+        }
+        if (bc.is_velocity) {
+#pragma unroll
+            for (int side = 2 * L::D - 1; side >= 0; --side) {
+                const int a = side >> 1;
+                const int inward = (side & 1) ? -1 : 1;
+                if (bc.type[side] != BC_VELOCITY || L::e(q, a) != inward) continue;
+                const int ghost = (side & 1) ? g.dmax[a] + 1 : g.dmin[a] - 1;
+                if (s[a] != ghost) continue;
+                if (!bc.vflag[side][plane_index(g, side, s)]) {
+                    // outside the inlet disc: bounce-back copy from the
+                    // boundary node next to the ghost (Solver.cu:621-633)
+                    int b[3] = {s[0], s[1], s[2]};
+                    b[a] = (side & 1) ? g.dmax[a] : g.dmin[a];
+                    return SP::ld(A, (long long)L::opp(q) * g.nodes + g.idx(b[0], b[1], b[2]), L::opp(q));
+                }
+                break;  // inside: the time-invariant inlet value stored in the ghost cell
+            }
+        }
+        return SP::ld(A, (long long)q * g.nodes + g.idx(s[0], s[1], s[2]), q);
+    }
+
+    __device__ CT lvl1(const int c[3], int q) const {
+        if (bc.is_wall) {
+#pragma unroll
+            for (int side = 0; side < 2 * L::D; ++side) {
+                const int a = side >> 1;
+                const int inward = (side & 1) ? -1 : 1;
+                const int t = bc.type[side];
+                if ((t == BC_WALL || t == BC_SLIP_WALL) && L::e(q, a) == inward &&
+                    c[a] == ((side & 1) ? g.dmax[a] : g.dmin[a]))
+                    return SP::ld(A, (long long)L::opp(q) * g.nodes + g.idx(c[0], c[1], c[2]), L::opp(q));
+            }
+        }
+        return raw(c, q);
+    }
+
+    __device__ CT lvl2(const int c[3], int q) const {
+        if (bc.is_fc) {
+#pragma unroll
+            for (int side = 2 * L::D - 1; side >= 0; --side) {
+                const int a = side >> 1;
+                const int inward = (side & 1) ? -1 : 1;
+                if (bc.type[side] != BC_FREE_CONVECT || L::e(q, a) != inward ||
+                    c[a] != ((side & 1) ? g.dmax[a] : g.dmin[a]))
+                    continue;
+                int nb[3] = {c[0], c[1], c[2]};
+                nb[a] += inward;
+                const CT cv = fabs(uold[(long long)a * g.nodes + g.idx(nb[0], nb[1], nb[2])]);
+                const CT prv = prev[side][(long long)q * plane_size(g, side) + plane_index(g, side, c)];
+                return (prv + cv * lvl1(nb, q)) / (CT(1) + cv);
+            }
+        }
+        return lvl1(c, q);
+    }
+
+    __device__ CT lvl3(const int c[3], int q) const {
+        if (bc.is_neumann) {
+#pragma unroll
+            for (int side = 2 * L::D - 1; side >= 0; --side) {
+                const int a = side >> 1;
+                const int inward = (side & 1) ? -1 : 1;
+                if (bc.type[side] != BC_NEUMANN || L::e(q, a) != inward ||
+                    c[a] != ((side & 1) ? g.dmax[a] : g.dmin[a]))
+                    continue;
+                int nb[3] = {c[0], c[1], c[2]};
+                nb[a] += inward;
+                return lvl2(nb, q);
+            }
+        }
+        return lvl2(c, q);
+    }
+};
+
+template <class L>
+__device__ inline bool on_edge(const DevGrid& g, const int c[3]) {
+    bool e = c[0] == g.dmin[0] || c[0] == g.dmax[0] || c[1] == g.dmin[1] || c[1] == g.dmax[1];
+    if (L::D == 3) e = e || c[2] == g.dmin[2] || c[2] == g.dmax[2];
+    return e;
+}
+
+// Writes the post-streaming value of an outflow boundary node into the
+// outflow plane of the next step (f_prev / g_prev, NsAllen.cu:850-854).
+template <class L, class CT>
+__device__ inline void store_prev(const DevGrid& g, const DevBC& bc, CT* const* prev_out, const int c[3], int q,
+                                  CT v) {
+#pragma unroll
+    for (int side = 0; side < 2 * L::D; ++side) {
+        const int a = side >> 1;
+        const int inward = (side & 1) ? -1 : 1;
+        if (bc.type[side] == BC_FREE_CONVECT && L::e(q, a) == inward &&
+            c[a] == ((side & 1) ? g.dmax[a] : g.dmin[a]))
+            prev_out[side][(long long)q * plane_size(g, side) + plane_index(g, side, c)] = v;
+    }
+}
+
+// ------------------------------------------------------------ node math
+// Velocity part of the equilibrium, Equilibrium_new (NsAllen.cu:77-94).
+template <class L, class CT>
+__device__ __forceinline__ CT gamma_u(int q, const CT u[3]) {
+    CT eu = CT(L::ex(q)) * u[0] + CT(L::ey(q)) * u[1];
+    CT u2 = u[0] * u[0] + u[1] * u[1];
+    if (L::D == 3) {
+        eu += CT(L::ez(q)) * u[2];
+        u2 += u[2] * u[2];
+    }
+    return CT(L::w(q)) * (CT(3) * eu + CT(4.5) * eu * eu - CT(1.5) * u2);
+}
+
+// gneq <- M^-1 S M gneq, stress moments scaled by s (NsAllen.cu:416-431).
+template <class L, class CT>
+__device__ __forceinline__ void wmrt_filter(CT v[L::Q], CT s) {
+    CT m[L::Q];
+#pragma unroll
+    for (int r = 0; r < L::Q; ++r) {
+        CT acc = CT(0);
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const int c = L::M(r, q);
+            if (c == 1) acc += v[q];
+            else if (c == -1) acc -= v[q];
+            else if (c != 0) acc += CT(c) * v[q];
+        }
+        m[r] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < L::NSTRESS; ++k) m[L::stress(k)] *= s;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        CT acc = CT(0);
+#pragma unroll
+        for (int r = 0; r < L::Q; ++r) {
+            const int c = L::minv_num(q, r);
+            if (c == 1) acc += m[r];
+            else if (c == -1) acc -= m[r];
+            else if (c != 0) acc += CT(c) * m[r];
+        }
+        v[q] = acc * (CT(1) / CT(L::minv_den(q)));
+    }
+}
+
+// Viscous force of calculateVF (NsAllen.cu:102-132).
+template <class L, class CT>
+__device__ __forceinline__ void viscous_force(const CT gneq[L::Q], CT tau, CT drho, const CT dc[3], CT fv[3]) {
+    CT sxx = 0, sxy = 0, syy = 0, szx = 0, szy = 0, szz = 0;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        const int x = L::ex(q), y = L::ey(q), z = L::ez(q);
+        if (x * x) sxx += gneq[q];
+        if (x * y) sxy += CT(x * y) * gneq[q];
+        if (y * y) syy += gneq[q];
+        if (L::D == 3) {
+            if (z * y) szy += CT(z * y) * gneq[q];
+            if (z * x) szx += CT(z * x) * gneq[q];
+            if (z * z) szz += gneq[q];
+        }
+    }
+    fv[0] = -tau * (sxx * dc[0] + sxy * dc[1] + szx * dc[2]) * drho;
+    fv[1] = -tau * (sxy * dc[0] + syy * dc[1] + szy * dc[2]) * drho;
+    fv[2] = (L::D == 3) ? -tau * (szz * dc[2] + szx * dc[0] + szy * dc[1]) * drho : CT(0);
+}
+
+template <class CT>
+__device__ __forceinline__ CT tau_of(const Phys<CT>& ph, CT phi) {
+    if (phi < CT(0)) return ph.tau_l;
+    if (phi > CT(1)) return ph.tau_h;
+    return ph.tau_l + phi * (ph.tau_h - ph.tau_l);
+}
+
+// Body force by case (NsAllen.cu:460-474); jet and unknown cases: 0.
+template <class CT>
+__device__ __forceinline__ void body_force(const Phys<CT>& ph, CT rho, CT fg[3]) {
+    CT base;
+    switch (ph.case_type) {
+        case CLIP_CASE_RTI: base = rho; break;
+        case CLIP_CASE_BUBBLE: base = rho - ph.rho_h; break;
+        case CLIP_CASE_DROP: base = rho - ph.rho_l; break;
+        default: base = CT(0); break;
+    }
+    fg[0] = base * ph.grav[0];
+    fg[1] = base * ph.grav[1];
+    fg[2] = base * ph.grav[2];
+}
+
+// phi stencil at interior node n: isotropic gradient (NsAllen.cu:280-322) and
+// Laplacian of the chemical potential (NsAllen.cu:207-236).
+template <class L, class CT>
+__device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const DevGrid& g, long long n, CT& c0,
+                                            CT dc[3], CT& lap) {
+    c0 = phi[n];
+    if (L::D == 2) {
+        const long long sy = g.sy;
+        const CT xp = phi[n + 1], xm = phi[n - 1], yp = phi[n + sy], ym = phi[n - sy];
+        const CT pp = phi[n + 1 + sy], pm = phi[n + 1 - sy], mp = phi[n - 1 + sy], mm = phi[n - 1 - sy];
+        dc[0] = (xp - xm) / CT(3) + (pm + pp - mm - mp) / CT(12);
+        dc[1] = (yp - ym) / CT(3) + (mp + pp - mm - pm) / CT(12);
+        dc[2] = CT(0);
+        lap = (mm + pm + mp + pp + CT(4) * (ym + xm + xp + yp) - CT(20) * c0) / CT(6);
+    } else {
+        const long long sy = g.sy, sz = g.sz;
+        CT v[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v[a][b][d] = phi[n + (a - 1) + (b - 1) * sy + (d - 1) * sz];
+        // gradient: faces 4/9, edges 1/9, corners 1/36, all times 1/2
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            CT face = 0, edge = 0, corner = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int off[3] = {a - 1, b - 1, d - 1};
+                        if (off[ax] == 0) continue;
+                        const int others = (off[0] != 0) + (off[1] != 0) + (off[2] != 0) - 1;
+                        const CT val = (off[ax] > 0) ? v[a][b][d] : -v[a][b][d];
+                        if (others == 0) face += val;
+                        else if (others == 1) edge += val;
+                        else corner += val;
+                    }
+            dc[ax] = CT(0.5) * (CT(4.0 / 9.0) * face + CT(1.0 / 9.0) * edge + CT(1.0 / 36.0) * corner);
+        }
+        CT faces = 0, edges = 0, corners = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int m = (a != 1) + (b != 1) + (d != 1);
+                    if (m == 1) faces += v[a][b][d];
+                    else if (m == 2) edges += v[a][b][d];
+                    else if (m == 3) corners += v[a][b][d];
+                }
+        lap = (CT(20) * faces + CT(6) * edges + corners - CT(200) * c0) / CT(48);
+    }
+}
+
+// ------------------------------------------------------------ kernels
+constexpr int BX = 64, BY = 4;
+
+// Thread -> node of the interior box shrunk by `inset` along each lattice axis.
+template <class L>
+__device__ __forceinline__ bool node_of_thread(const DevGrid& g, int inset, int c[3]) {
+    c[0] = blockIdx.x * BX + threadIdx.x + g.dmin[0] + inset;
+    c[1] = blockIdx.y * BY + threadIdx.y + g.dmin[1] + inset;
+    c[2] = blockIdx.z + g.dmin[2] + (L::D == 3 ? inset : 0);
+    return c[0] <= g.dmax[0] - inset && c[1] <= g.dmax[1] - inset;
+}
+
+// Number of interior nodes that touch the ghost ring (the boundary layer).
+template <class L>
+__host__ __device__ inline long long shell_count(const DevGrid& g) {
+    const long long nx = g.dmax[0] - g.dmin[0] + 1, ny = g.dmax[1] - g.dmin[1] + 1;
+    const long long ring = 2 * nx + 2 * (ny - 2);
+    if (L::D == 2) return ring;
+    const long long nz = g.dmax[2] - g.dmin[2] + 1;
+    return 2 * nx * ny + (nz - 2) * ring;
+}
+
+// Thread t -> t-th boundary-layer node: in 3D the two z faces, then the ring
+// of every inner z plane; in 2D the ring of the single plane.
+template <class L>
+__device__ __forceinline__ bool shell_node(const DevGrid& g, int c[3]) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = g.dmax[0] - g.dmin[0] + 1, ny = g.dmax[1] - g.dmin[1] + 1;
+    c[2] = g.dmin[2];
+    if (L::D == 3) {
+        const long long plane = (long long)nx * ny;
+        if (t < 2 * plane) {
+            c[2] = (t < plane) ? g.dmin[2] : g.dmax[2];
+            const long long r = t % plane;
+            c[0] = g.dmin[0] + (int)(r % nx);
+            c[1] = g.dmin[1] + (int)(r / nx);
+            return true;
+        }
+        t -= 2 * plane;
+        const long long ring = 2LL * nx + 2LL * (ny - 2);
+        const long long kk = t / ring;
+        if (kk >= g.dmax[2] - g.dmin[2] - 1) return false;
+        c[2] = g.dmin[2] + 1 + (int)kk;
+        t -= kk * ring;
+    } else if (t >= 2LL * nx + 2LL * (ny - 2)) {
+        return false;
+    }
+    if (t < nx) {
+        c[0] = g.dmin[0] + (int)t;
+        c[1] = g.dmin[1];
+    } else if (t < 2 * nx) {
+        c[0] = g.dmin[0] + (int)(t - nx);
+        c[1] = g.dmax[1];
+    } else {
+        const long long r = t - 2 * nx;
+        c[0] = (r < ny - 2) ? g.dmin[0] : g.dmax[0];
+        c[1] = g.dmin[1] + 1 + (int)((r < ny - 2) ? r : r - (ny - 2));
+    }
+    return true;
+}
+
+// Node selection of the step kernels: SHELL = boundary layer (resolved pulls),
+// !SHELL && !DIRECT = inner box (plain offset pulls), DIRECT = every interior node.
+template <class L, bool DIRECT, bool SHELL>
+__device__ __forceinline__ bool select_node(const DevGrid& g, int c[3]) {
+    if (SHELL) return shell_node<L>(g, c);
+    return node_of_thread<L>(g, DIRECT ? 0 : 1, c);
+}
+
+// phi^{n+1} = sum_q g_post (kernelMacroscopicg, NsAllen.cu:516-540).
+template <class L, class ST, class CT, class SP, bool DIRECT, bool SHELL>
+__global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
+    int c[3];
+    if (!select_node<L, DIRECT, SHELL>(a.g, c)) return;
+    const long long n = a.g.idx(c[0], c[1], c[2]);
+    CT sum = CT(0);
+    if (DIRECT) {
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q);
+    } else if (!SHELL) {
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const long long src = n - L::ex(q) - L::ey(q) * a.g.sy - L::ez(q) * a.g.sz;
+            sum += SP::ld(a.gin, (long long)q * a.g.nodes + src, q);
+        }
+    } else {
+        Puller<L, ST, CT, SP> P{a.gin, a.g, a.bc, a.uin, a.prev_g_in};
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) sum += P.lvl3(c, q);
+    }
+    a.phi[n] = sum;
+}
+
+enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
+
+// Fused macroscopic + collision (kernelMacroscopicf NsAllen.cu:369-511,
+// normal_FD 241-275, kernelCollisionMRTg 545-584, kernelCollisionMRTf 589-763).
+template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL>
+__global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
+    constexpr int Q = L::Q;
+    int c[3];
+    if (!select_node<L, DIRECT, SHELL>(a.g, c)) return;
+    const DevGrid& g = a.g;
+    const Phys<CT>& ph = a.ph;
+    const long long n = g.idx(c[0], c[1], c[2]);
+    // outflow planes are written only by boundary-layer nodes
+    const bool edge = SHELL || (DIRECT && on_edge<L>(g, c));
+
+    // phi stencil -> grad phi, chemical potential
+    CT c0, dc[3], lap;
+    phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap);
+    const CT mu = CT(4) * ph.beta * c0 * (c0 - CT(1)) * (c0 - CT(0.5)) - ph.kappa * lap;
+    const CT rho = ph.rho_l + c0 * (ph.rho_h - ph.rho_l);
+    const CT drho = ph.rho_h - ph.rho_l;
+    const CT drho3 = drho / CT(3);
+    const CT tau = tau_of(ph, c0);
+    const CT s = CT(1) / (tau + CT(0.5));
+
+    // f_post
+    CT fp[Q];
+    {
+        Puller<L, ST, CT, SP> P{a.fin, g, a.bc, a.uin, a.prev_f_in};
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q);
+            else if (!SHELL) {
+                const long long src = n - L::ex(q) - L::ey(q) * g.sy - L::ez(q) * g.sz;
+                fp[q] = SP::ld(a.fin, (long long)q * g.nodes + src, q);
+            } else
+                fp[q] = P.lvl3(c, q);
+        }
+        if (a.bc.is_fc && edge) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fp[q]);
+        }
+    }
+
+    CT fg[3];
+    body_force(ph, rho, fg);
+
+    CT p, u[3];
+    if (MODE == MODE_FULL) {
+        // macroscopic p, u (kernelMacroscopicf): gneq from u_old
+        p = CT(0);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) p += fp[q];
+        CT uo[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) uo[d] = (d < L::D) ? a.uin[(long long)d * g.nodes + n] : CT(0);
+        CT gneq[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) gneq[q] = fp[q] - (p * CT(L::w(q)) + gamma_u<L, CT>(q, uo));
+        wmrt_filter<L, CT>(gneq, s);
+        CT fv[3];
+        viscous_force<L, CT>(gneq, tau, drho, dc, fv);
+        CT F[3], mom[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if (L::ex(q)) mom[0] += CT(L::ex(q)) * fp[q];
+            if (L::ey(q)) mom[1] += CT(L::ey(q)) * fp[q];
+            if (L::ez(q)) mom[2] += CT(L::ez(q)) * fp[q];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? mom[d] + CT(0.5) * F[d] / rho : CT(0);
+#pragma unroll
+        for (int d = 0; d < L::D; ++d) a.uout[(long long)d * g.nodes + n] = u[d];
+    } else {
+        p = a.p[n];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? a.ucol[(long long)d * g.nodes + n] : CT(0);
+    }
+
+    // normal (normal_FD)
+    CT nr[3];
+    {
+        CT sq = dc[0] * dc[0] + dc[1] * dc[1];
+        if (L::D == 3) sq += dc[2] * dc[2];
+        const CT t = sqrt(sq) + CT(1e-32);
+        nr[0] = dc[0] / t;
+        nr[1] = dc[1] / t;
+        nr[2] = (L::D == 3) ? dc[2] / t : CT(0);
+    }
+
+    // g collision (kernelCollisionMRTg), one population at a time
+    {
+        const CT wc = CT(1) / (CT(0.5) + CT(3) * ph.mobility);
+        const CT aterm = (CT(3) * ph.mobility * (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5))))) / ph.W;
+        Puller<L, ST, CT, SP> P{a.gin, g, a.bc, a.uin, a.prev_g_in};
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            CT gp;
+            if (DIRECT) gp = SP::ld(a.gin, (long long)q * g.nodes + n, q);
+            else if (!SHELL) {
+                const long long src = n - L::ex(q) - L::ey(q) * g.sy - L::ez(q) * g.sz;
+                gp = SP::ld(a.gin, (long long)q * g.nodes + src, q);
+            } else
+                gp = P.lvl3(c, q);
+            if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
+            CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
+            if (L::D == 3) en += CT(L::ez(q)) * nr[2];
+            const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);
+            SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc);
+        }
+    }
+
+    // f collision (kernelCollisionMRTf): two filtered passes
+    {
+        CT ga[Q], gneq[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            ga[q] = gamma_u<L, CT>(q, u);
+            gneq[q] = fp[q] - (p * CT(L::w(q)) + ga[q]);
+        }
+        wmrt_filter<L, CT>(gneq, s);
+        CT fv[3];
+        viscous_force<L, CT>(gneq, tau, drho, dc, fv);
+        CT F[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
+        CT hlp[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            CT eF = CT(L::ex(q)) * F[0] + CT(L::ey(q)) * F[1];
+            if (L::D == 3) eF += CT(L::ez(q)) * F[2];
+            hlp[q] = CT(3) * CT(L::w(q)) * eF / rho;
+            gneq[q] = fp[q] - (p * CT(L::w(q)) + ga[q] - CT(0.5) * hlp[q]);
+        }
+        wmrt_filter<L, CT>(gneq, s);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) SP::st(a.fout, (long long)q * g.nodes + n, q, fp[q] - gneq[q] + hlp[q]);
+    }
+}
+
+// ------------------------------------------------------------ phi ghost ring
+// Visits every cell of the ghost planes of the first 2*D sides (edge cells
+// more than once; every visit writes the same value).
+__device__ inline bool ghost_cell_of_thread(const DevGrid& g, int D, long long t, int c[3]) {
+    for (int side = 0; side < 2 * D; ++side) {
+        const long long ps = plane_size(g, side);
+        if (t < ps) {
+            const int a = side >> 1;
+            const int t1 = (a == 0) ? 1 : 0;
+            const int t2 = (a == 2) ? 1 : 2;
+            c[t1] = (int)(t % g.ext[t1]);
+            c[t2] = (int)(t / g.ext[t1]);
+            c[a] = (side & 1) ? g.dmax[a] + 1 : g.dmin[a] - 1;
+            return true;
+        }
+        t -= ps;
+    }
+    return false;
+}
+
+// Periodic ghost copy (Solver.cu:75-141) and/or mirror copy
+// (Solver.cu:927-978) of phi: ghost <- phi at the cell with every periodic
+// axis wrapped and every mirror axis clamped.  The source cell is never
+// written by the same launch.
+template <class CT>
+__global__ void k_ghost_resolve(CT* phi, DevGrid g, DevBC bc, int D, long long total, int do_periodic,
+                                int do_mirror) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int c[3];
+    if (!ghost_cell_of_thread(g, D, t, c)) return;
+    int s[3] = {c[0], c[1], c[2]};
+    bool changed = false;
+    for (int a = 0; a < D; ++a) {
+        const bool lo = c[a] < g.dmin[a], hi = c[a] > g.dmax[a];
+        if (!lo && !hi) continue;
+        if (do_periodic && bc.periodic[a]) {
+            s[a] = lo ? g.dmax[a] : g.dmin[a];
+            changed = true;
+        } else if (do_mirror && bc.mirror_gate) {
+            const int t = bc.type[2 * a + (hi ? 1 : 0)];
+            if (t == BC_WALL || t == BC_FREE_CONVECT || t == BC_NEUMANN || t == BC_SLIP_WALL || t == BC_DO_NOTHING) {
+                s[a] = lo ? g.dmin[a] : g.dmax[a];
+                changed = true;
+            }
+        }
+    }
+    if (changed) phi[g.idx(c[0], c[1], c[2])] = phi[g.idx(s[0], s[1], s[2])];
+}
+
+// Inlet side, outside the disc: phi ghost <- boundary node (Solver.cu:621-633).
+template <class CT>
+__global__ void k_ghost_velocity(CT* phi, DevGrid g, const unsigned char* flag, int side) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= plane_size(g, side)) return;
+    const int a = side >> 1;
+    const int t1 = (a == 0) ? 1 : 0;
+    const int t2 = (a == 2) ? 1 : 2;
+    int c[3];
+    c[t1] = (int)(t % g.ext[t1]);
+    c[t2] = (int)(t / g.ext[t1]);
+    c[a] = (side & 1) ? g.dmax[a] + 1 : g.dmin[a] - 1;
+    if (flag[t]) return;
+    int b[3] = {c[0], c[1], c[2]};
+    b[a] = (side & 1) ? g.dmax[a] : g.dmin[a];
+    phi[g.idx(c[0], c[1], c[2])] = phi[g.idx(b[0], b[1], b[2])];
+}
+
+// ------------------------------------------------------------ initialisation
+// KernelInitializeDistributions with the interior stencils of
+// deviceInitializer (NsAllen.cu:137-202, 897-908).  All nodes, both buffers.
+template <class L, class ST, class CT, class SP>
+__global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const DevGrid& g = a.g;
+    if (n >= g.nodes) return;
+    int c[3];
+    c[0] = (int)(n % g.ext[0]);
+    c[1] = (int)((n / g.ext[0]) % g.ext[1]);
+    c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
+    const Phys<CT>& ph = a.ph;
+    const CT c0 = a.phi[n];
+    const CT rho = ph.rho_l + c0 * (ph.rho_h - ph.rho_l);
+    CT p = CT(0);
+    if (ph.case_type == CLIP_CASE_BUBBLE) p = CT(0) - c0 * sigma / L2 / (rho / CT(3));
+    else if (ph.case_type == CLIP_CASE_DROP) p = CT(0) + c0 * sigma / L2 / (rho / CT(3));
+    a.p[n] = p;
+    for (int d = 0; d < L::D; ++d) a.uout[(long long)d * g.nodes + n] = CT(0);
+    CT nr[3] = {0, 0, 0};
+    bool interior = true;
+    for (int d = 0; d < L::D; ++d) interior = interior && c[d] >= g.dmin[d] && c[d] <= g.dmax[d];
+    if (interior) {
+        CT cc, dc[3], lap;
+        phi_stencil<L, CT>(a.phi, g, n, cc, dc, lap);
+        CT sq = dc[0] * dc[0] + dc[1] * dc[1];
+        if (L::D == 3) sq += dc[2] * dc[2];
+        const CT t = sqrt(sq) + CT(1e-32);
+        for (int d = 0; d < L::D; ++d) nr[d] = dc[d] / t;
+    }
+    const CT u0[3] = {0, 0, 0};
+    const CT hw = (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5)))) / ph.W;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        const CT ga = gamma_u<L, CT>(q, u0);
+        CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
+        if (L::D == 3) en += CT(L::ez(q)) * nr[2];
+        const CT hlp = CT(L::w(q)) * (hw * en);
+        const CT gv = c0 * (ga + CT(L::w(q))) - CT(0.5) * hlp;
+        const CT fv = p * CT(L::w(q)) + ga;
+        const long long i = (long long)q * g.nodes + n;
+        SP::st(a.fout, i, q, fv);
+        SP::st(f2, i, q, fv);
+        SP::st(a.gout, i, q, gv);
+        SP::st(g2, i, q, gv);
+        if (a.bc.is_fc && interior) {
+            store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fv);
+            store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gv);
+        }
+    }
+}
+
+// Inlet populations inside the disc (JetBoundary, Solver.cu:578-619 and its
+// y/z counterparts).  Every input is a ghost-cell value that no phase after
+// initialisation rewrites, so the result is time-invariant and is computed
+// once, into both buffers.
+template <class L, class ST, class CT, class SP>
+__global__ void k_inlet_init(DevGrid g, const unsigned char* flag, int side, const CT* phi, ST* fA, ST* gA, ST* fB,
+                             ST* gB, const CT* Uplane) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= plane_size(g, side)) return;
+    if (!flag[t]) return;
+    const int ax = side >> 1;
+    const int t1 = (ax == 0) ? 1 : 0;
+    const int t2 = (ax == 2) ? 1 : 2;
+    int c[3];
+    c[t1] = (int)(t % g.ext[t1]);
+    c[t2] = (int)(t / g.ext[t1]);
+    c[ax] = (side & 1) ? g.dmax[ax] + 1 : g.dmin[ax] - 1;
+    const long long n = g.idx(c[0], c[1], c[2]);
+    const CT U[3] = {Uplane[3 * t + 0], Uplane[3 * t + 1], Uplane[3 * t + 2]};
+    const CT cg = phi[n];
+    const int inward = (side & 1) ? -1 : 1;
+    CT feq[L::Q], geq[L::Q], Mt[3] = {0, 0, 0}, N = 0;
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+        const CT fa = gamma_u<L, CT>(q, U);
+        feq[q] = fa;
+        geq[q] = cg * (fa + CT(L::w(q)));
+        if (L::e(q, ax) == 0) {
+            const CT fq = SP::ld(fA, (long long)q * g.nodes + n, q);
+            const CT gq = SP::ld(gA, (long long)q * g.nodes + n, q);
+            for (int d = 0; d < L::D; ++d)
+                if (d != ax) Mt[d] += CT(L::e(q, d)) * (fq - feq[q]);
+            N += gq - geq[q];
+        }
+    }
+    feq[0] = geq[0] = CT(0);
+    const CT weight = CT(L::NIN - 1);
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+        if (L::e(q, ax) != inward) continue;
+        const int b = L::opp(q);
+        bool normal = true;
+        CT corr = 0;
+        for (int d = 0; d < L::D; ++d)
+            if (d != ax) {
+                corr += CT(L::e(q, d)) * Mt[d];
+                if (L::e(q, d) != 0) normal = false;
+            }
+        const CT fb = SP::ld(fA, (long long)b * g.nodes + n, b);
+        const CT gb = SP::ld(gA, (long long)b * g.nodes + n, b);
+        const CT fv = fb + feq[q] - feq[b] - (CT(1) / weight) * corr;
+        CT gv = geq[q] - (gb - geq[b]);
+        if (!normal) gv -= N / weight;
+        const long long i = (long long)q * g.nodes + n;
+        SP::st(fA, i, q, fv);
+        SP::st(fB, i, q, fv);
+        SP::st(gA, i, q, gv);
+        SP::st(gB, i, q, gv);
+    }
+}
+
+// ------------------------------------------------------------ exports
+// Post-streaming populations of every interior node (and p = sum f_post)
+// into a dense CT array; ghost nodes copy the stored value.
+template <class L, class ST, class CT, class SP, bool DIRECT>
+__global__ void k_export_post(StepArgs<ST, CT> a, bool is_f, CT* out, CT* pout) {
+    const ST* src = is_f ? a.fin : a.gin;
+    const CT* const* prev = is_f ? a.prev_f_in : a.prev_g_in;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const DevGrid& g = a.g;
+    if (n >= g.nodes) return;
+    int c[3];
+    c[0] = (int)(n % g.ext[0]);
+    c[1] = (int)((n / g.ext[0]) % g.ext[1]);
+    c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
+    bool interior = true;
+    for (int d = 0; d < L::D; ++d) interior = interior && c[d] >= g.dmin[d] && c[d] <= g.dmax[d];
+    Puller<L, ST, CT, SP> P{src, g, a.bc, a.uin, prev};
+    CT sum = 0;
+    for (int q = 0; q < L::Q; ++q) {
+        CT v;
+        if (!interior || DIRECT) v = SP::ld(src, (long long)q * g.nodes + n, q);
+        else v = P.lvl3(c, q);
+        out[(long long)q * g.nodes + n] = v;
+        sum += v;
+    }
+    if (pout && interior) pout[n] = sum;
+}
+
+// Stored populations into a dense CT array.
+template <class ST, class CT, class SP>
+__global__ void k_export_raw(const ST* src, CT* out, long long count, long long nodes) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    out[t] = SP::ld(src, t, (int)(t / nodes));
+}
+
+template <class ST, class CT, class SP>
+__global__ void k_import_raw(ST* dst, const CT* in, long long count, long long nodes) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    SP::st(dst, t, (int)(t / nodes), in[t]);
+}
+
+// mu, grad phi and normal at interior nodes (ghosts left at 0).
+template <class L, class CT>
+__global__ void k_export_stencil(const CT* phi, DevGrid g, Phys<CT> ph, CT* mu, CT* dphi, CT* normal) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodes) return;
+    int c[3];
+    c[0] = (int)(n % g.ext[0]);
+    c[1] = (int)((n / g.ext[0]) % g.ext[1]);
+    c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
+    bool interior = true;
+    for (int d = 0; d < L::D; ++d) interior = interior && c[d] >= g.dmin[d] && c[d] <= g.dmax[d];
+    if (!interior) {
+        mu[n] = 0;
+        for (int d = 0; d < L::D; ++d) dphi[d * g.nodes + n] = normal[d * g.nodes + n] = 0;
+        return;
+    }
+    CT c0, dc[3], lap;
+    phi_stencil<L, CT>(phi, g, n, c0, dc, lap);
+    mu[n] = CT(4) * ph.beta * c0 * (c0 - CT(1)) * (c0 - CT(0.5)) - ph.kappa * lap;
+    CT sq = dc[0] * dc[0] + dc[1] * dc[1];
+    if (L::D == 3) sq += dc[2] * dc[2];
+    const CT t = sqrt(sq) + CT(1e-32);
+    for (int d = 0; d < L::D; ++d) {
+        dphi[d * g.nodes + n] = dc[d];
+        normal[d * g.nodes + n] = dc[d] / t;
+    }
+}
+
+// ------------------------------------------------------------ reductions
+// FP64 sum and non-finite count of a scalar field over interior nodes
+// (Reporter::sum, Reporter.cu:50-114): wave64 shuffles, then LDS across the
+// block's waves, one partial per block; a second launch folds the partials.
+template <class CT>
+__global__ __launch_bounds__(256) void k_reduce_interior(const CT* x, DevGrid g, int D, double* part,
+                                                         unsigned* nonfinite, double ref) {
+    __shared__ double ws[4];
+    __shared__ unsigned wn[4];
+    const long long nx = g.dmax[0] - g.dmin[0] + 1;
+    const long long ny = g.dmax[1] - g.dmin[1] + 1;
+    const long long nz = g.dmax[2] - g.dmin[2] + 1;
+    const long long total = nx * ny * nz;
+    double acc = 0.0;
+    unsigned bad = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx) + g.dmin[0];
+        const int j = (int)((t / nx) % ny) + g.dmin[1];
+        const int k = (int)(t / (nx * ny)) + g.dmin[2];
+        const double v = (double)x[g.idx(i, j, k)];
+        if (!isfinite(v)) bad++;
+        else acc += v - ref;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc += __shfl_xor(acc, off, 64);
+        bad += __shfl_xor(bad, off, 64);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        ws[wave] = acc;
+        wn[wave] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        unsigned b = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            s += ws[w];
+            b += wn[w];
+        }
+        part[blockIdx.x] = s;
+        if (b) atomicAdd(nonfinite, b);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nparts, double* out) {
+    __shared__ double ws[4];
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < nparts; t += blockDim.x) acc += part[t];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w];
+        out[0] = s;
+    }
+}
+
+}  // namespace clipamd

@@ -1,0 +1,209 @@
+/*
+ * clip_amd.h — C-ABI of the MI355X-native coupled two-phase LBM step
+ * (Allen–Cahn phase field + weighted-MRT Navier–Stokes, D2Q9 / D3Q19).
+ *
+ * Every entry point returns CLIP_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is returned by clip_last_error().
+ * No C++ exception crosses this boundary.  A handle owns its device memory;
+ * caller buffers stay caller-owned.  A handle is not re-entrant: calls on one
+ * handle must be serialised by the caller.
+ *
+ * The reference exposes a C++ class surface instead of a C-ABI; each entry
+ * point below names the reference member it replaces (paths relative to the
+ * CLIP source tree).
+ */
+#ifndef CLIP_AMD_H
+#define CLIP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLIP_MAX_DIM 3
+#define CLIP_NUM_SIDES 6
+#define CLIP_MAX_GEOMETRIES 16
+#define CLIP_MAX_INLET_MODES 8
+
+/* ---- status codes ------------------------------------------------------ */
+#define CLIP_OK 0
+#define CLIP_ERR_ARG (-1)       /* invalid argument / config value          */
+#define CLIP_ERR_PARSE (-2)     /* config.txt syntax or missing key/side    */
+#define CLIP_ERR_HIP (-3)       /* HIP runtime failure                      */
+#define CLIP_ERR_IO (-4)        /* file missing, truncated or mismatched    */
+#define CLIP_ERR_STATE (-5)     /* call out of order (e.g. step before init)*/
+#define CLIP_ERR_DIVERGED (-6)  /* NaN/Inf found by the health check        */
+#define CLIP_ERR_NOMEM (-7)     /* device allocation failed                 */
+
+/* ---- enums (values equal the reference's enums) ------------------------- */
+/* InputData::CaseType, include/input/InputData.cuh:56-63 */
+enum clip_case_type {
+    CLIP_CASE_BUBBLE = 0, CLIP_CASE_DROP = 1, CLIP_CASE_RTI = 2,
+    CLIP_CASE_JET = 3, CLIP_CASE_UNKNOWN = 4
+};
+/* Boundary::Objects, include/boundary/Boundary.cuh:66-76 */
+enum clip_side {
+    CLIP_X_MINUS = 0, CLIP_X_PLUS = 1, CLIP_Y_MINUS = 2,
+    CLIP_Y_PLUS = 3, CLIP_Z_MINUS = 4, CLIP_Z_PLUS = 5
+};
+/* Boundary::Type, include/boundary/Boundary.cuh:81-92 */
+enum clip_bc_type {
+    CLIP_BC_WALL = 0, CLIP_BC_SLIP_WALL = 1, CLIP_BC_FREE_CONVECT = 2,
+    CLIP_BC_PERIODIC = 3, CLIP_BC_NEUMANN = 4, CLIP_BC_VELOCITY = 5,
+    CLIP_BC_DO_NOTHING = 6, CLIP_BC_UNKNOWN = 7
+};
+/* Geometry::Type, include/geometry/Geometry.cuh:54-62 */
+enum clip_geometry_type {
+    CLIP_GEOM_CIRCLE = 0, CLIP_GEOM_SPHERE = 1, CLIP_GEOM_SQUARE = 2,
+    CLIP_GEOM_CUBE = 3, CLIP_GEOM_PERTURBATION = 4, CLIP_GEOM_UNKNOWN = 5
+};
+
+/* Storage / arithmetic policy of a solver handle. */
+enum clip_precision {
+    CLIP_PREC_F64 = 0,        /* FP64 populations, FP64 arithmetic (drop-in)     */
+    CLIP_PREC_F32 = 1,        /* FP32 populations, FP32 arithmetic               */
+    CLIP_PREC_MIXED_F16 = 2,  /* FP16 population deviations, FP32 arithmetic     */
+    CLIP_PREC_MIXED_BF16 = 3  /* BF16 population deviations, FP32 arithmetic     */
+};
+
+/* Fields reachable through clip_get_field / clip_set_field.  Names follow the
+ * reference's DataArray::deviceDataArray (include/data/DataArray.cuh:234-250). */
+enum clip_field {
+    CLIP_FIELD_PHI = 0,    /* dev_c      (1 dof)   */
+    CLIP_FIELD_RHO = 1,    /* dev_rho    (1 dof)   */
+    CLIP_FIELD_P = 2,      /* dev_p      (1 dof)   */
+    CLIP_FIELD_U = 3,      /* dev_vel    (DIM dof) */
+    CLIP_FIELD_MU = 4,     /* dev_mu     (1 dof)   */
+    CLIP_FIELD_DPHI = 5,   /* dev_dc     (DIM dof) */
+    CLIP_FIELD_NORMAL = 6, /* dev_normal (DIM dof) */
+    CLIP_FIELD_F = 7,      /* dev_f      (Q dof)   */
+    CLIP_FIELD_G = 8,      /* dev_g      (Q dof)   */
+    CLIP_FIELD_F_POST = 9, /* dev_f_post (Q dof)   */
+    CLIP_FIELD_G_POST = 10 /* dev_g_post (Q dof)   */
+};
+
+/* Host-buffer layouts. */
+enum clip_layout {
+    CLIP_LAYOUT_AOS = 0,  /* CLIP's ((i*ext_y+j)*ext_z+k)*ndof+dof, ghosts included
+                             (include/domain/Domain.cuh:63-68)                       */
+    CLIP_LAYOUT_SOA = 1   /* ((dof*ext_z+k)*ext_y+j)*ext_x+i, ghosts included       */
+};
+
+/* ---- configuration ------------------------------------------------------
+ * Mirrors InputData::SimParams (include/input/InputData.cuh:69-116), the
+ * Boundary::BCTypeMap (Boundary.cuh:109-113) and Geometry::GeometryDevice
+ * (Geometry.cuh:81-90) in one POD struct.  Derived members are filled by
+ * clip_config_derive() (InputData.cu:76-114). */
+typedef struct clip_config {
+    int dim;                       /* 2 or 3 (the reference's ENABLE_2D/3D)    */
+    int case_type;                 /* enum clip_case_type                      */
+    unsigned final_step, output_interval, report_interval;
+    unsigned checkpoint_interval, checkpoint_copy;
+    double t_final;
+    double N[CLIP_MAX_DIM];        /* 2D: interior size; 3D: size incl. ghosts */
+    double reference_length, reference_velocity;
+    double Re, We, Pe, Ca, Bo, Mo;
+    double rho_ratio, mu_ratio;
+    double gravity[CLIP_MAX_DIM];  /* body-force acceleration                  */
+    double interface_width, mobility, amplitude;
+    /* optional overrides (key present => flag set) */
+    int has_sigma, has_mu_h, has_gravity_scale;
+    double sigma_override, mu_h_override, gravity_scale;
+    /* derived */
+    double rho_h, rho_l, sigma, mu_h, mu_l, tau_h, tau_l, beta, kappa;
+    /* boundary blocks */
+    int bc_type[CLIP_NUM_SIDES];
+    int bc_present[CLIP_NUM_SIDES];
+    double bc_value[CLIP_NUM_SIDES][CLIP_MAX_DIM];
+    /* geometry blocks */
+    int num_geometries;
+    int geom_type[CLIP_MAX_GEOMETRIES];
+    int geom_id[CLIP_MAX_GEOMETRIES];
+    double geom_center[CLIP_MAX_GEOMETRIES][CLIP_MAX_DIM];
+    double geom_length[CLIP_MAX_GEOMETRIES][CLIP_MAX_DIM];
+    double geom_radius[CLIP_MAX_GEOMETRIES];
+    double geom_amplitude[CLIP_MAX_GEOMETRIES];
+    /* inlet modulation (extension, no reference counterpart):
+       u_in = value * (1 + sum_m amp[m] cos((m+1)*theta + phase[m])),
+       theta measured around geometry 0's centre in the inlet plane. */
+    int inlet_modes;
+    double inlet_mode_amp[CLIP_MAX_INLET_MODES];
+    double inlet_mode_phase[CLIP_MAX_INLET_MODES];
+} clip_config;
+
+/* Sizes of a created solver. */
+typedef struct clip_info {
+    int dim, q;
+    int precision;
+    unsigned ext[CLIP_MAX_DIM];       /* extents incl. ghosts (Domain.cu:22-82) */
+    unsigned long long nodes;         /* ext_x*ext_y*ext_z                      */
+    unsigned long long interior;      /* interior (updated) nodes               */
+    unsigned long long device_bytes;  /* device memory owned by the handle      */
+    unsigned long long step;          /* TimeInfo::simInfo::currentStep         */
+} clip_info;
+
+typedef struct clip_solver clip_solver;
+
+/* ---- configuration (host only) ----------------------------------------- */
+/* Parse CLIP's config.txt: replaces InputData::read_config (InputData.cu:57-115),
+ * Boundary::readBoundaries (Boundary.cu:146-279) and Geometry::readGeometries
+ * (Geometry.cu:72-182).  dim_override = 2 or 3 forces the dimension (the
+ * reference's compile-time switch); 0 infers 2 when N[2] <= 1. */
+int clip_config_from_file(const char* path, int dim_override, clip_config* out);
+/* Same, from a string holding the file's text. */
+int clip_config_from_string(const char* text, int dim_override, clip_config* out);
+/* Recompute derived parameters (InputData.cu:76-114) and validate. */
+int clip_config_derive(clip_config* cfg);
+
+/* ---- solver lifecycle --------------------------------------------------- */
+/* Replaces the object graph built in main.cu:61-65 (Domain, DataArray,
+ * NSAllen): allocates every device buffer for the configuration. */
+int clip_create(const clip_config* cfg, int precision, clip_solver** out);
+/* Bind the handle to a HIP stream (hipStream_t passed as void*; NULL = the
+ * handle's own stream). */
+int clip_set_stream(clip_solver* s, void* hip_stream);
+/* NSAllen::initialCondition + DataArray::updateDevice + NSAllen::deviceInitializer
+ * (NsAllen.cu:768-824, DataArray.cu:124-128, NsAllen.cu:897-908). */
+int clip_init_fresh(clip_solver* s);
+/* nsteps x NSAllen::solve (NsAllen.cu:881-892), TimeInfo::increment each. */
+int clip_step(clip_solver* s, int nsteps);
+/* Block until the handle's stream has drained. */
+int clip_synchronize(clip_solver* s);
+/* Copy a field to host memory (double, layout as requested, ghosts included;
+ * host_out holds ndof*nodes doubles).  Replaces DataArray::updateHost
+ * (DataArray.cu:131-138). */
+int clip_get_field(clip_solver* s, int field, int layout, double* host_out);
+/* Overwrite PHI, U, P, F, G, F_POST or G_POST from host memory (same shape as
+ * clip_get_field).  The pending collision is recomputed afterwards. */
+int clip_set_field(clip_solver* s, int field, int layout, const double* host_in);
+/* FP64 sum of a scalar field over interior nodes: Reporter::sum
+ * (Reporter.cu:95-114).  field = PHI or RHO. */
+int clip_sum(clip_solver* s, int field, double* out);
+/* Health check of Reporter::print (Reporter.cu:124-144): sums PHI and returns
+ * CLIP_ERR_DIVERGED (instead of throwing) when phi holds a NaN or Inf. */
+int clip_check_health(clip_solver* s, double* phi_sum);
+/* CheckPointer::save / load (CheckPointer.cu:53-102): CLIP's file set in
+ * `dir` (dev_f.bin, dev_g.bin, dev_f_post.bin, dev_g_post.bin,
+ * [dev_f_prev.bin, dev_g_prev.bin], timeInfo.bin, domainSize.bin, info.txt)
+ * plus clip_amd_state.bin (u, phi with ghosts, outflow planes) so that a
+ * resume is bit-exact.  Loading without the extension file reproduces the
+ * reference's macroscopic() restart with u_old = 0. */
+int clip_checkpoint_save(clip_solver* s, const char* dir);
+int clip_checkpoint_load(clip_solver* s, const char* dir);
+/* CheckPointer::rotateFolders (CheckPointer.cu:108-138). */
+int clip_checkpoint_rotate(const char* base, unsigned copies);
+int clip_get_info(const clip_solver* s, clip_info* out);
+/* Time of the device work of the last clip_step call, measured with HIP events
+ * around each kernel class on the handle's stream: total, and the three step
+ * kernels (ghost refresh, phase sum, fused collide-stream). */
+int clip_last_step_timing(const clip_solver* s, double* total_ms, double* ghost_ms,
+                          double* phase_ms, double* fused_ms);
+int clip_enable_timing(clip_solver* s, int on);
+void clip_destroy(clip_solver* s);
+const char* clip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLIP_AMD_H */

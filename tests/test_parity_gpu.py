@@ -1,0 +1,127 @@
+"""FP64 HIP path vs the FP64 CPU oracle on identical inputs.
+
+Bar: relative L2 <= 1e-11 on phi, u and p (interior nodes) after the given
+number of steps; populations compared too.  Runs through the C-ABI
+(libclip_amd.so via ctypes).
+"""
+import numpy as np
+import pytest
+
+from clip_amd import Config, Solver
+from tests import cases
+from tests.oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(b.ravel())
+    num = np.linalg.norm((a - b).ravel())
+    return num / den if den > 0 else num
+
+
+def run_pair(text, steps, precision="f64"):
+    cfg = Config.from_string(text)
+    print(f"case dim={cfg.dim} N={list(cfg.N)} bc={list(cfg.bc_type)} steps={steps}", flush=True)
+    gpu = Solver(cfg, precision).init_fresh()
+    orc = Oracle(cfg).init()
+    if steps:
+        gpu.step(steps)
+        orc.step(steps)
+    return cfg, gpu, orc
+
+
+def assert_fields(gpu, orc, fields=("phi", "u", "p"), tol=TOL):
+    errs = {}
+    for f in fields:
+        a = gpu.interior(f)
+        b = orc.interior(f)
+        assert np.all(np.isfinite(a)), f
+        errs[f] = rel_l2(a, b)
+    print("rel-L2:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for f, e in errs.items():
+        assert e <= tol, f"{f}: rel-L2 {e:.3e} > {tol}"
+    return errs
+
+
+def test_init_matches_oracle():
+    cfg, gpu, orc = run_pair(cases.drop2d(n=32, radius=8), 0)
+    assert_fields(gpu, orc, ("phi", "p", "rho"))
+    for f in ("f", "g"):
+        assert rel_l2(gpu.get(f), orc.get(f)) <= 1e-14
+
+
+def test_drop2d_small_periodic():
+    cfg, gpu, orc = run_pair(cases.drop2d(n=32, radius=8), 200)
+    assert_fields(gpu, orc, ("phi", "u", "p", "rho", "mu", "dphi"))
+    for f in ("f", "g", "f_post", "g_post"):
+        e = rel_l2(gpu.interior(f), orc.interior(f))
+        assert e <= TOL, (f, e)
+
+
+def test_drop2d_c1_parity_gate():
+    """Parity gate: 128x128 periodic drop, R=25, rhoRatio 1000, 1000 steps."""
+    cfg, gpu, orc = run_pair(cases.drop2d(n=128, radius=25), 1000)
+    assert_fields(gpu, orc)
+    s_gpu = gpu.sum("phi")
+    s_orc = orc.interior("phi").sum()
+    assert abs(s_gpu - s_orc) <= 1e-9 * abs(s_orc)
+
+
+def test_rti2d_walls():
+    cfg, gpu, orc = run_pair(cases.rti2d(32, 64), 300)
+    assert_fields(gpu, orc)
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL
+
+
+def test_jet2d_inlet_outflow():
+    cfg, gpu, orc = run_pair(cases.jet2d(24, 32, radius=4), 200)
+    assert_fields(gpu, orc)
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL
+
+
+@pytest.mark.parametrize("sides", [
+    [("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall")],
+    [("x-", "neumann"), ("x+", "neumann"), ("y-", "wall"), ("y+", "free convect")],
+    [("x-", "periodic"), ("x+", "periodic"), ("y-", "slip wall"), ("y+", "do nothing")],
+    [("x-", "periodic"), ("x+", "periodic"), ("y-", "wall"), ("y+", "neumann")],
+    [("x-", "wall"), ("x+", "do nothing"), ("y-", "free convect"), ("y+", "periodic")],
+])
+def test_box2d_boundaries(sides):
+    cfg, gpu, orc = run_pair(cases.box2d(sides), 150)
+    assert_fields(gpu, orc)
+
+
+def test_bubble3d_periodic():
+    cfg, gpu, orc = run_pair(cases.bubble3d(n=16, radius=4), 100)
+    assert_fields(gpu, orc)
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL
+
+
+def test_jet3d_inlet_outflow():
+    cfg, gpu, orc = run_pair(cases.jet3d((16, 20, 16), radius=3), 100)
+    assert_fields(gpu, orc)
+
+
+def test_jet3d_zminus_neumann_outlet():
+    cfg, gpu, orc = run_pair(cases.jet3d((14, 14, 20), radius=3, inlet_side="z-", outlet_type="neumann"), 80)
+    assert_fields(gpu, orc)
+
+
+def test_box3d_walls():
+    sides = [("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall"), ("z-", "periodic"), ("z+", "periodic")]
+    cfg, gpu, orc = run_pair(cases.bubble3d(n=14, radius=3, rho_ratio=10, mu_ratio=10, sides=sides), 80)
+    assert_fields(gpu, orc)
+
+
+def test_sum_matches_interior_sum():
+    cfg, gpu, orc = run_pair(cases.bubble3d(n=12, radius=3), 10)
+    phi = gpu.interior("phi")
+    assert abs(gpu.sum("phi") - phi.sum()) <= 1e-12 * abs(phi.sum())
+    rho = gpu.interior("rho")
+    assert abs(gpu.sum("rho") - rho.sum()) <= 1e-10 * abs(rho.sum())
