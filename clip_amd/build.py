@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libclip_amd.so")
 DRIVER = os.path.join(HERE, "clip")
 
-HIP_SOURCES = ["solver.hip"]
+# (source, extra flags): the FP64 instantiation is built without FP contraction
+HIP_SOURCES = [("solver.hip", []), ("inst_f64.hip", ["-ffp-contract=off"]), ("inst_f32.hip", [])]
 CPP_SOURCES = ["config.cpp"]
 ARCH = os.environ.get("CLIP_AMD_ARCH", "gfx950")
 
@@ -40,9 +41,9 @@ def build(verbose=True):
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     objs = []
-    for src in HIP_SOURCES:
+    for src, extra in HIP_SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        run([hipcc(), "-c", *COMMON, *DEVICE_FLAGS, os.path.join(CSRC, src), "-o", obj])
+        run([hipcc(), "-c", *COMMON, *DEVICE_FLAGS, *extra, os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CPP_SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")

@@ -1,0 +1,66 @@
+// clip_main.cpp — the `clip` executable with the reference driver's shape
+// (src/main.cu:39-91): read ./config.txt, fresh start or `-resume`, then the
+// step loop with reporting and checkpoints.  Differences: no VTS output (the
+// reference copies fields to the host every step for it, VTSwriter.cu:72-79),
+// and optional `-precision f64|f32`, `-config <path>`, `-steps <n>` flags.
+#include "clip_amd.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+int main(int argc, char** argv) {
+    bool resume = false;
+    std::string config = "config.txt";
+    int precision = CLIP_PREC_F64;
+    long long steps_override = -1;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a(argv[i]);
+        if (a == "-resume") resume = true;
+        else if (a == "-config" && i + 1 < argc) config = argv[++i];
+        else if (a == "-steps" && i + 1 < argc) steps_override = std::atoll(argv[++i]);
+        else if (a == "-precision" && i + 1 < argc) {
+            const std::string p(argv[++i]);
+            precision = (p == "f32") ? CLIP_PREC_F32 : (p == "mixed_f16") ? CLIP_PREC_MIXED_F16
+                      : (p == "mixed_bf16") ? CLIP_PREC_MIXED_BF16 : CLIP_PREC_F64;
+        }
+    }
+    try {
+        clip::InputData inputData(config);
+        clip::Domain domain(inputData);
+        clip::Geometry geometry(inputData);
+        clip::Boundary boundary(inputData, domain);
+        clip::NSAllen ns(inputData, domain, boundary, geometry, precision);
+        clip::TimeInfo timeInfo(inputData);
+        clip::Reporter reporter(ns, inputData, timeInfo);
+        clip::CheckPointer checkpointer(ns, inputData, timeInfo);
+        if (resume) {
+            checkpointer.load();
+        } else {
+            ns.initialCondition();
+            ns.deviceInitializer();
+        }
+        const long long final_step = steps_override >= 0 ? timeInfo.getCurrentStep() + steps_override
+                                                         : (long long)timeInfo.getFinalStep();
+        const auto t0 = std::chrono::steady_clock::now();
+        long long done = 0;
+        while (timeInfo.getCurrentStep() < final_step) {
+            ns.solve();
+            timeInfo.increment();
+            ++done;
+            reporter.print();
+            checkpointer.save();
+        }
+        clip::check(clip_synchronize(ns.handle()));
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        clip_info inf;
+        clip::check(clip_get_info(ns.handle(), &inf));
+        if (done > 0)
+            std::printf("steps %lld  time %.3f s  MLUPS %.1f\n", done, sec, (double)inf.interior * done / sec / 1e6);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

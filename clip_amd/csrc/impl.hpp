@@ -1,0 +1,59 @@
+// impl.hpp — type-erased solver interface shared by the C-ABI translation
+// unit and the per-precision instantiations.
+#pragma once
+
+#include "clip_amd.h"
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace clipamd {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define HIP_CHECK(expr)                                                                               \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
+                           std::to_string(__LINE__) + ")");                                           \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Type-erased interface over the precision/lattice instantiations.
+struct Impl {
+    virtual ~Impl() {}
+    virtual void init_fresh() = 0;
+    virtual void step(int n) = 0;
+    virtual void get_field(int field, std::vector<double>& out) = 0;  // SoA, ghosts included
+    virtual void set_field(int field, const std::vector<double>& in) = 0;
+    virtual void resume(const std::vector<double>& f, const std::vector<double>& g, const std::vector<double>& fpost,
+                        const std::vector<double>& gpost, const std::vector<double>* phi,
+                        const std::vector<double>* uold) = 0;
+    virtual void uold_soa(std::vector<double>& out) = 0;
+    virtual double reduce(int field, unsigned* nonfinite) = 0;
+    virtual void sync() = 0;
+    virtual void set_stream(hipStream_t s) = 0;
+    virtual void timing(double* total, double* ghost, double* phase, double* fused) const = 0;
+    virtual void enable_timing(bool on) = 0;
+    int D = 2, Q = 9;
+    unsigned long long bytes = 0;
+    long long step_count = 0;
+    bool initialized = false;
+    bool direct_old = false;  // pop[1-cur] holds f_post directly
+    std::vector<double> direct_f_interior, direct_g_interior;  // dev_f/dev_g interior after a resume
+};
+
+// Factories, one per translation unit (the FP64 one is compiled without
+// floating-point contraction so that it reproduces the oracle bit for bit).
+std::unique_ptr<Impl> make_impl_f64(const clip_config& c);
+std::unique_ptr<Impl> make_impl_f32(const clip_config& c);
+
+}  // namespace clipamd

@@ -87,10 +87,21 @@ struct StorePlain {
 };
 
 // ------------------------------------------------------------ arguments
+// Boundary layer = interior nodes on a plane next to a non-periodic side.
+// The planes of the first non-periodic axis are taken whole, those of later
+// axes exclude the rows already covered.  The host fills `axes` (ascending)
+// and the node count of each axis' plane pair.
+struct ShellDesc {
+    int na;
+    int axes[3];
+    long long count[3];
+};
+
 template <class ST, class CT>
 struct StepArgs {
     DevGrid g;
     DevBC bc;
+    ShellDesc shell;
     Phys<CT> ph;
     const ST* fin;   // f* of the buffer being pulled
     const ST* gin;
@@ -267,7 +278,8 @@ __device__ __forceinline__ void wmrt_filter(CT v[L::Q], CT s) {
             else if (c == -1) acc -= m[r];
             else if (c != 0) acc += CT(c) * m[r];
         }
-        v[q] = acc * (CT(1) / CT(L::minv_den(q)));
+        if constexpr (sizeof(CT) == 8) v[q] = acc / CT(L::minv_den(q));
+        else v[q] = acc * (CT(1) / CT(L::minv_den(q)));
     }
 }
 
@@ -321,13 +333,16 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
                                             CT dc[3], CT& lap) {
     c0 = phi[n];
     if (L::D == 2) {
+        // sums in lexicographic (x offset, then y offset) order
         const long long sy = g.sy;
         const CT xp = phi[n + 1], xm = phi[n - 1], yp = phi[n + sy], ym = phi[n - sy];
         const CT pp = phi[n + 1 + sy], pm = phi[n + 1 - sy], mp = phi[n - 1 + sy], mm = phi[n - 1 - sy];
-        dc[0] = (xp - xm) / CT(3) + (pm + pp - mm - mp) / CT(12);
-        dc[1] = (yp - ym) / CT(3) + (mp + pp - mm - pm) / CT(12);
+        dc[0] = (xp - xm) / CT(3) + ((pm + pp) - (mm + mp)) / CT(12);
+        dc[1] = (yp - ym) / CT(3) + ((mp + pp) - (mm + pm)) / CT(12);
         dc[2] = CT(0);
-        lap = (mm + pm + mp + pp + CT(4) * (ym + xm + xp + yp) - CT(20) * c0) / CT(6);
+        const CT corners = ((mm + mp) + pm) + pp;
+        const CT edges = ((xm + ym) + yp) + xp;
+        lap = (corners + CT(4) * edges - CT(20) * c0) / CT(6);
     } else {
         const long long sy = g.sy, sz = g.sz;
         CT v[3][3][3];
@@ -337,10 +352,11 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
             for (int b = 0; b < 3; ++b)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) v[a][b][d] = phi[n + (a - 1) + (b - 1) * sy + (d - 1) * sz];
-        // gradient: faces 4/9, edges 1/9, corners 1/36, all times 1/2
+        // gradient: faces 4/9, edges 1/9, corners 1/36, all times 1/2; the
+        // +side and -side sums accumulate in lexicographic (x, y, z) order
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            CT face = 0, edge = 0, corner = 0;
+            CT plus[3] = {0, 0, 0}, minus[3] = {0, 0, 0};
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -350,12 +366,11 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
                         const int off[3] = {a - 1, b - 1, d - 1};
                         if (off[ax] == 0) continue;
                         const int others = (off[0] != 0) + (off[1] != 0) + (off[2] != 0) - 1;
-                        const CT val = (off[ax] > 0) ? v[a][b][d] : -v[a][b][d];
-                        if (others == 0) face += val;
-                        else if (others == 1) edge += val;
-                        else corner += val;
+                        if (off[ax] > 0) plus[others] += v[a][b][d];
+                        else minus[others] += v[a][b][d];
                     }
-            dc[ax] = CT(0.5) * (CT(4.0 / 9.0) * face + CT(1.0 / 9.0) * edge + CT(1.0 / 36.0) * corner);
+            dc[ax] = CT(0.5) * (CT(4.0 / 9.0) * (plus[0] - minus[0]) + CT(1.0 / 9.0) * (plus[1] - minus[1]) +
+                                CT(1.0 / 36.0) * (plus[2] - minus[2]));
         }
         CT faces = 0, edges = 0, corners = 0;
 #pragma unroll
@@ -376,88 +391,121 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
 // ------------------------------------------------------------ kernels
 constexpr int BX = 64, BY = 4;
 
-// Thread -> node of the interior box shrunk by `inset` along each lattice axis.
+// Thread -> node of the bulk region: the interior box shrunk by one node
+// along every non-periodic lattice axis (inset = 1), or the whole interior
+// (inset = 0, used by the non-pulling kernels).
 template <class L>
-__device__ __forceinline__ bool node_of_thread(const DevGrid& g, int inset, int c[3]) {
-    c[0] = blockIdx.x * BX + threadIdx.x + g.dmin[0] + inset;
-    c[1] = blockIdx.y * BY + threadIdx.y + g.dmin[1] + inset;
-    c[2] = blockIdx.z + g.dmin[2] + (L::D == 3 ? inset : 0);
-    return c[0] <= g.dmax[0] - inset && c[1] <= g.dmax[1] - inset;
+__device__ __forceinline__ bool node_of_thread(const DevGrid& g, const DevBC& bc, int inset, int c[3]) {
+    const int ix = inset && !bc.periodic[0], iy = inset && !bc.periodic[1];
+    const int iz = (L::D == 3) ? (inset && !bc.periodic[2]) : 0;
+    c[0] = blockIdx.x * BX + threadIdx.x + g.dmin[0] + ix;
+    c[1] = blockIdx.y * BY + threadIdx.y + g.dmin[1] + iy;
+    c[2] = blockIdx.z + g.dmin[2] + iz;
+    return c[0] <= g.dmax[0] - ix && c[1] <= g.dmax[1] - iy;
 }
 
-// Number of interior nodes that touch the ghost ring (the boundary layer).
-template <class L>
-__host__ __device__ inline long long shell_count(const DevGrid& g) {
-    const long long nx = g.dmax[0] - g.dmin[0] + 1, ny = g.dmax[1] - g.dmin[1] + 1;
-    const long long ring = 2 * nx + 2 * (ny - 2);
-    if (L::D == 2) return ring;
-    const long long nz = g.dmax[2] - g.dmin[2] + 1;
-    return 2 * nx * ny + (nz - 2) * ring;
-}
 
-// Thread t -> t-th boundary-layer node: in 3D the two z faces, then the ring
-// of every inner z plane; in 2D the ring of the single plane.
 template <class L>
-__device__ __forceinline__ bool shell_node(const DevGrid& g, int c[3]) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nx = g.dmax[0] - g.dmin[0] + 1, ny = g.dmax[1] - g.dmin[1] + 1;
-    c[2] = g.dmin[2];
-    if (L::D == 3) {
-        const long long plane = (long long)nx * ny;
-        if (t < 2 * plane) {
-            c[2] = (t < plane) ? g.dmin[2] : g.dmax[2];
-            const long long r = t % plane;
-            c[0] = g.dmin[0] + (int)(r % nx);
-            c[1] = g.dmin[1] + (int)(r / nx);
-            return true;
+__host__ __device__ inline ShellDesc make_shell(const DevGrid& g, const DevBC& bc) {
+    ShellDesc sd{};
+    long long n[3];
+    for (int a = 0; a < 3; ++a) n[a] = g.dmax[a] - g.dmin[a] + 1;
+    for (int a = 0; a < L::D; ++a) {
+        if (bc.periodic[a]) continue;
+        long long cnt = 2;
+        for (int b = 0; b < 3; ++b) {
+            if (b == a) continue;
+            bool earlier = false;
+            for (int k = 0; k < sd.na; ++k) earlier |= sd.axes[k] == b;
+            cnt *= earlier ? (n[b] - 2) : n[b];
         }
-        t -= 2 * plane;
-        const long long ring = 2LL * nx + 2LL * (ny - 2);
-        const long long kk = t / ring;
-        if (kk >= g.dmax[2] - g.dmin[2] - 1) return false;
-        c[2] = g.dmin[2] + 1 + (int)kk;
-        t -= kk * ring;
-    } else if (t >= 2LL * nx + 2LL * (ny - 2)) {
-        return false;
+        sd.axes[sd.na] = a;
+        sd.count[sd.na] = cnt;
+        sd.na++;
     }
-    if (t < nx) {
-        c[0] = g.dmin[0] + (int)t;
-        c[1] = g.dmin[1];
-    } else if (t < 2 * nx) {
-        c[0] = g.dmin[0] + (int)(t - nx);
-        c[1] = g.dmax[1];
-    } else {
-        const long long r = t - 2 * nx;
-        c[0] = (r < ny - 2) ? g.dmin[0] : g.dmax[0];
-        c[1] = g.dmin[1] + 1 + (int)((r < ny - 2) ? r : r - (ny - 2));
+    return sd;
+}
+
+template <class L>
+__device__ __forceinline__ bool shell_node(const DevGrid& g, const ShellDesc& sd, int c[3]) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = 0; k < sd.na; ++k) {
+        if (t >= sd.count[k]) {
+            t -= sd.count[k];
+            continue;
+        }
+        const int a = sd.axes[k];
+        const long long half = sd.count[k] / 2;
+        const bool hi = t >= half;
+        long long r = hi ? t - half : t;
+        for (int b = 0; b < 3; ++b) {
+            if (b == a) continue;
+            bool earlier = false;
+            for (int m = 0; m < k; ++m) earlier |= sd.axes[m] == b;
+            const int lo = g.dmin[b] + (earlier ? 1 : 0);
+            const long long size = (long long)(g.dmax[b] - g.dmin[b] + 1) - (earlier ? 2 : 0);
+            c[b] = lo + (int)(r % size);
+            r /= size;
+        }
+        c[a] = hi ? g.dmax[a] : g.dmin[a];
+        return true;
     }
-    return true;
+    return false;
+}
+
+// Source index of population q pulled by a bulk node: n - e_q, wrapped along
+// periodic axes (kernelPeriodicBoundary + kernelStreaming, Solver.cu:75-141,
+// NsAllen.cu:327-364).
+struct PullOffsets {
+    long long m[3], p[3];  // offset for e = +1 (source at -1) and e = -1 (source at +1)
+};
+template <class L>
+__device__ __forceinline__ PullOffsets pull_offsets(const DevGrid& g, const int c[3]) {
+    PullOffsets o;
+    const long long st[3] = {1, g.sy, g.sz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long long span = (long long)(g.dmax[a] - g.dmin[a]) * st[a];
+        o.m[a] = (c[a] == g.dmin[a]) ? span : -st[a];
+        o.p[a] = (c[a] == g.dmax[a]) ? -span : st[a];
+    }
+    return o;
+}
+template <class L>
+__device__ __forceinline__ long long pull_src(long long n, const PullOffsets& o, int q) {
+    long long s = n;
+    if (L::ex(q) > 0) s += o.m[0];
+    if (L::ex(q) < 0) s += o.p[0];
+    if (L::ey(q) > 0) s += o.m[1];
+    if (L::ey(q) < 0) s += o.p[1];
+    if (L::ez(q) > 0) s += o.m[2];
+    if (L::ez(q) < 0) s += o.p[2];
+    return s;
 }
 
 // Node selection of the step kernels: SHELL = boundary layer (resolved pulls),
-// !SHELL && !DIRECT = inner box (plain offset pulls), DIRECT = every interior node.
+// !SHELL && !DIRECT = bulk (offset pulls, periodic wrap), DIRECT = every
+// interior node.
 template <class L, bool DIRECT, bool SHELL>
-__device__ __forceinline__ bool select_node(const DevGrid& g, int c[3]) {
-    if (SHELL) return shell_node<L>(g, c);
-    return node_of_thread<L>(g, DIRECT ? 0 : 1, c);
+__device__ __forceinline__ bool select_node(const DevGrid& g, const DevBC& bc, const ShellDesc& sd, int c[3]) {
+    if (SHELL) return shell_node<L>(g, sd, c);
+    return node_of_thread<L>(g, bc, DIRECT ? 0 : 1, c);
 }
 
 // phi^{n+1} = sum_q g_post (kernelMacroscopicg, NsAllen.cu:516-540).
 template <class L, class ST, class CT, class SP, bool DIRECT, bool SHELL>
 __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
     int c[3];
-    if (!select_node<L, DIRECT, SHELL>(a.g, c)) return;
+    if (!select_node<L, DIRECT, SHELL>(a.g, a.bc, a.shell, c)) return;
     const long long n = a.g.idx(c[0], c[1], c[2]);
     CT sum = CT(0);
     if (DIRECT) {
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q);
     } else if (!SHELL) {
+        const PullOffsets po = pull_offsets<L>(a.g, c);
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) {
-            const long long src = n - L::ex(q) - L::ey(q) * a.g.sy - L::ez(q) * a.g.sz;
-            sum += SP::ld(a.gin, (long long)q * a.g.nodes + src, q);
-        }
+        for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + pull_src<L>(n, po, q), q);
     } else {
         Puller<L, ST, CT, SP> P{a.gin, a.g, a.bc, a.uin, a.prev_g_in};
 #pragma unroll
@@ -474,7 +522,7 @@ template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHE
 __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     constexpr int Q = L::Q;
     int c[3];
-    if (!select_node<L, DIRECT, SHELL>(a.g, c)) return;
+    if (!select_node<L, DIRECT, SHELL>(a.g, a.bc, a.shell, c)) return;
     const DevGrid& g = a.g;
     const Phys<CT>& ph = a.ph;
     const long long n = g.idx(c[0], c[1], c[2]);
@@ -491,6 +539,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     const CT tau = tau_of(ph, c0);
     const CT s = CT(1) / (tau + CT(0.5));
 
+    PullOffsets po;
+    if (!DIRECT && !SHELL) po = pull_offsets<L>(g, c);
+
     // f_post
     CT fp[Q];
     {
@@ -498,10 +549,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q);
-            else if (!SHELL) {
-                const long long src = n - L::ex(q) - L::ey(q) * g.sy - L::ez(q) * g.sz;
-                fp[q] = SP::ld(a.fin, (long long)q * g.nodes + src, q);
-            } else
+            else if (!SHELL)
+                fp[q] = SP::ld(a.fin, (long long)q * g.nodes + pull_src<L>(n, po, q), q);
+            else
                 fp[q] = P.lvl3(c, q);
         }
         if (a.bc.is_fc && edge) {
@@ -567,10 +617,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         for (int q = 0; q < Q; ++q) {
             CT gp;
             if (DIRECT) gp = SP::ld(a.gin, (long long)q * g.nodes + n, q);
-            else if (!SHELL) {
-                const long long src = n - L::ex(q) - L::ey(q) * g.sy - L::ez(q) * g.sz;
-                gp = SP::ld(a.gin, (long long)q * g.nodes + src, q);
-            } else
+            else if (!SHELL)
+                gp = SP::ld(a.gin, (long long)q * g.nodes + pull_src<L>(n, po, q), q);
+            else
                 gp = P.lvl3(c, q);
             if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
             CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
@@ -644,15 +693,17 @@ __global__ void k_ghost_resolve(CT* phi, DevGrid g, DevBC bc, int D, long long t
     for (int a = 0; a < D; ++a) {
         const bool lo = c[a] < g.dmin[a], hi = c[a] > g.dmax[a];
         if (!lo && !hi) continue;
-        if (do_periodic && bc.periodic[a]) {
+        // the mirror copy runs after the periodic copy, so on a mirror-type
+        // side it decides the value (NsAllen.cu:884-887)
+        const int t = bc.type[2 * a + (hi ? 1 : 0)];
+        const bool mirror = bc.mirror_gate && (t == BC_WALL || t == BC_FREE_CONVECT || t == BC_NEUMANN ||
+                                               t == BC_SLIP_WALL || t == BC_DO_NOTHING);
+        if (do_mirror && mirror) {
+            s[a] = lo ? g.dmin[a] : g.dmax[a];
+            changed = true;
+        } else if (do_periodic && bc.periodic[a] && !(do_mirror && mirror)) {
             s[a] = lo ? g.dmax[a] : g.dmin[a];
             changed = true;
-        } else if (do_mirror && bc.mirror_gate) {
-            const int t = bc.type[2 * a + (hi ? 1 : 0)];
-            if (t == BC_WALL || t == BC_FREE_CONVECT || t == BC_NEUMANN || t == BC_SLIP_WALL || t == BC_DO_NOTHING) {
-                s[a] = lo ? g.dmin[a] : g.dmax[a];
-                changed = true;
-            }
         }
     }
     if (changed) phi[g.idx(c[0], c[1], c[2])] = phi[g.idx(s[0], s[1], s[2])];
@@ -907,15 +958,16 @@ __global__ __launch_bounds__(256) void k_reduce_interior(const CT* x, DevGrid g,
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nparts, double* out) {
-    __shared__ double ws[4];
-    double acc = 0.0;
+template <class T>
+__global__ __launch_bounds__(256) void k_reduce_final(const T* part, int nparts, T* out) {
+    __shared__ T ws[4];
+    T acc = 0.0;
     for (int t = threadIdx.x; t < nparts; t += blockDim.x) acc += part[t];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0.0;
+        T s = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w];
         out[0] = s;
     }

@@ -1,0 +1,576 @@
+// solver_impl.hpp — SolverImpl<Lattice, StorageT, ComputeT, StorePolicy>:
+// device buffers of one handle and the launch sequence of every entry point.
+//
+// State of a handle between calls (after clip_init_fresh or clip_step):
+//   pop[cur]   f*, g* of the pending collision (the next step pulls it)
+//   pop[1-cur] f*, g* the last step pulled (the reference's dev_f / dev_g);
+//              right after initialisation or a resume it holds f_post directly
+//   u[ucur]    u of the last macroscopic step; u[1-ucur] the one before
+//   prev[pcur] outflow planes read by the next pull
+//   phi        phi with its ghost ring as the last step's stencils saw it
+// so one step is: ghost refresh, k_phase, k_fused (FULL), swap.
+#pragma once
+
+#include "geometry.hpp"
+#include "impl.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace clipamd {
+
+template <class L, class ST, class CT, class SP>
+struct SolverImpl : Impl {
+    static constexpr int QL = L::Q;
+    clip_config cfg;
+    DevGrid g;
+    DevBC bc;
+    Phys<CT> ph;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    ST* fbuf[2] = {nullptr, nullptr};
+    ST* gbuf[2] = {nullptr, nullptr};
+    CT* phi = nullptr;
+    CT* u[2] = {nullptr, nullptr};
+    CT* p = nullptr;
+    CT* prevf[2][6] = {};
+    CT* prevg[2][6] = {};
+    unsigned char* vflag[6] = {};
+    CT* scratch = nullptr;        // export staging (max(Q, D) * nodes)
+    double* red_part = nullptr;   // reduction partials
+    unsigned* red_bad = nullptr;
+    double* red_out = nullptr;
+    int cur = 1, ucur = 0, pcur = 0;
+    bool timing_on = false;
+    hipEvent_t ev[8] = {};
+    double t_total = 0, t_ghost = 0, t_phase = 0, t_fused = 0;
+    std::vector<void*> allocations;
+
+    template <class T>
+    T* alloc(size_t count) {
+        void* ptr = nullptr;
+        if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        allocations.push_back(ptr);
+        bytes += count * sizeof(T);
+        return (T*)ptr;
+    }
+
+    explicit SolverImpl(const clip_config& c) : cfg(c) {
+        D = L::D;
+        Q = L::Q;
+        if (c.dim != L::D) throw ConfigError("lattice/dimension mismatch");
+        // Extents: 2D N+2 with the ghost ring outside N, 3D N with the ghost
+        // ring inside N (Domain.cu:22-82).
+        for (int a = 0; a < 3; ++a) {
+            if (L::D == 2) {
+                g.ext[a] = (a < 2) ? (int)c.N[a] + 2 : 1;
+                g.dmin[a] = (a < 2) ? 1 : 0;
+                g.dmax[a] = (a < 2) ? g.ext[a] - 2 : 0;
+            } else {
+                g.ext[a] = (int)c.N[a];
+                g.dmin[a] = 1;
+                g.dmax[a] = g.ext[a] - 2;
+            }
+        }
+        g.sy = g.ext[0];
+        g.sz = (long long)g.ext[0] * g.ext[1];
+        g.nodes = g.sz * g.ext[2];
+        for (int s = 0; s < 6; ++s) bc.type[s] = (s < 2 * L::D) ? c.bc_type[s] : -1;
+        bool isWall = false, isFC = false, isNeu = false, isVel = false, isSlip = false;
+        for (int s = 0; s < 2 * L::D; ++s) {
+            isWall |= c.bc_type[s] == CLIP_BC_WALL;
+            isSlip |= c.bc_type[s] == CLIP_BC_SLIP_WALL;
+            isFC |= c.bc_type[s] == CLIP_BC_FREE_CONVECT;
+            isNeu |= c.bc_type[s] == CLIP_BC_NEUMANN;
+            isVel |= c.bc_type[s] == CLIP_BC_VELOCITY;
+        }
+        for (int a = 0; a < 3; ++a)
+            bc.periodic[a] = (a < L::D) && (c.bc_type[2 * a] == CLIP_BC_PERIODIC || c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC);
+        bc.is_wall = isWall;
+        bc.is_fc = isFC;
+        bc.is_neumann = isNeu;
+        bc.is_velocity = isVel;
+        bc.mirror_gate = isWall || isFC || isSlip || isNeu;
+        ph.rho_h = (CT)c.rho_h;
+        ph.rho_l = (CT)c.rho_l;
+        ph.tau_h = (CT)c.tau_h;
+        ph.tau_l = (CT)c.tau_l;
+        ph.beta = (CT)c.beta;
+        ph.kappa = (CT)c.kappa;
+        ph.mobility = (CT)c.mobility;
+        ph.W = (CT)c.interface_width;
+        for (int d = 0; d < 3; ++d) ph.grav[d] = (d < L::D) ? (CT)c.gravity[d] : CT(0);
+        ph.case_type = c.case_type;
+
+        HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+        const size_t nodes = (size_t)g.nodes;
+        for (int b = 0; b < 2; ++b) {
+            fbuf[b] = alloc<ST>(nodes * QL);
+            gbuf[b] = alloc<ST>(nodes * QL);
+            u[b] = alloc<CT>(nodes * L::D);
+        }
+        phi = alloc<CT>(nodes);
+        p = alloc<CT>(nodes);
+        scratch = alloc<CT>(nodes * std::max(QL, L::D));
+        red_part = alloc<double>(1024);
+        red_bad = alloc<unsigned>(1);
+        red_out = alloc<double>(1);
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (c.bc_type[s] == CLIP_BC_FREE_CONVECT)
+                for (int b = 0; b < 2; ++b) {
+                    prevf[b][s] = alloc<CT>(plane_size(g, s) * QL);
+                    prevg[b][s] = alloc<CT>(plane_size(g, s) * QL);
+                }
+            if (c.bc_type[s] == CLIP_BC_VELOCITY) {
+                const long long ps = plane_size(g, s);
+                std::vector<unsigned char> fl(ps);
+                const int a = s / 2;
+                const int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+                for (long long t = 0; t < ps; ++t) {
+                    int cc[3];
+                    cc[t1] = (int)(t % g.ext[t1]);
+                    cc[t2] = (int)(t / g.ext[t1]);
+                    cc[a] = (s & 1) ? g.dmax[a] + 1 : g.dmin[a] - 1;
+                    const double z = (L::D == 3) ? (double)cc[2] : 0.0;
+                    fl[t] = sdf(cfg, 0, cc[0], cc[1], z) <= 0 ? 1 : 0;
+                }
+                vflag[s] = alloc<unsigned char>(ps);
+                HIP_CHECK(hipMemcpy(vflag[s], fl.data(), ps, hipMemcpyHostToDevice));
+            }
+            bc.vflag[s] = vflag[s];
+        }
+        for (int s = 2 * L::D; s < 6; ++s) bc.vflag[s] = nullptr;
+        HIP_CHECK(hipMemsetAsync(phi, 0, nodes * sizeof(CT), stream));
+        HIP_CHECK(hipMemsetAsync(p, 0, nodes * sizeof(CT), stream));
+        for (int b = 0; b < 2; ++b) HIP_CHECK(hipMemsetAsync(u[b], 0, nodes * L::D * sizeof(CT), stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    ~SolverImpl() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* ptr : allocations) (void)hipFree(ptr);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    void set_stream(hipStream_t s) override {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (s == nullptr) {
+            if (!own_stream) {
+                HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+                own_stream = true;
+            }
+            return;
+        }
+        if (own_stream) HIP_CHECK(hipStreamDestroy(stream));
+        stream = s;
+        own_stream = false;
+    }
+
+    void sync() override { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+    StepArgs<ST, CT> args(int pull_buf, int uin, int prev_in) {
+        StepArgs<ST, CT> a;
+        a.g = g;
+        a.bc = bc;
+        a.shell = make_shell<L>(g, bc);
+        a.ph = ph;
+        a.fin = fbuf[pull_buf];
+        a.gin = gbuf[pull_buf];
+        a.fout = fbuf[1 - pull_buf];
+        a.gout = gbuf[1 - pull_buf];
+        a.phi = phi;
+        a.uin = u[uin];
+        a.uout = u[1 - uin];
+        a.ucol = u[uin];
+        a.p = p;
+        for (int s = 0; s < 6; ++s) {
+            a.prev_f_in[s] = prevf[prev_in][s];
+            a.prev_f_out[s] = prevf[1 - prev_in][s];
+            a.prev_g_in[s] = prevg[prev_in][s];
+            a.prev_g_out[s] = prevg[1 - prev_in][s];
+        }
+        return a;
+    }
+
+    // Grid over the interior box shrunk by `inset` along each non-periodic
+    // lattice axis (the bulk region of the pulling kernels when inset = 1).
+    dim3 grid_box(int inset) const {
+        int n[3];
+        for (int a = 0; a < 3; ++a) {
+            const int in = (a < L::D && inset && !bc.periodic[a]) ? 1 : 0;
+            n[a] = g.dmax[a] - g.dmin[a] + 1 - 2 * in;
+        }
+        if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return dim3(0, 0, 0);
+        return dim3((n[0] + BX - 1) / BX, (n[1] + BY - 1) / BY, n[2]);
+    }
+    dim3 grid_interior() const { return grid_box(0); }
+    long long shell_nodes() const {
+        const ShellDesc sd = make_shell<L>(g, bc);
+        long long t = 0;
+        for (int k = 0; k < sd.na; ++k) t += sd.count[k];
+        return t;
+    }
+    unsigned shell_blocks() const { return (unsigned)((shell_nodes() + BX * BY - 1) / (BX * BY)); }
+
+    // Bulk (inner box) and boundary-layer launches of a pulling step kernel.
+    template <int MODE>
+    void launch_fused_pull(const StepArgs<ST, CT>& a) {
+        const dim3 gb = grid_box(1);
+        if (gb.x) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, dim3(BX, BY), 0, stream>>>(a);
+        if (shell_blocks()) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
+        launch_check("k_fused");
+    }
+    void launch_phase_pull(const StepArgs<ST, CT>& a) {
+        const dim3 gb = grid_box(1);
+        if (gb.x) k_phase<L, ST, CT, SP, false, false><<<gb, dim3(BX, BY), 0, stream>>>(a);
+        if (shell_blocks()) k_phase<L, ST, CT, SP, false, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
+        launch_check("k_phase");
+    }
+
+    void launch_check(const char* what) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw HipError(std::string("launch of ") + what + ": " + hipGetErrorString(e));
+    }
+
+    void ghost_refresh() {
+        long long total = 0;
+        for (int s = 0; s < 2 * L::D; ++s) total += plane_size(g, s);
+        const int tb = 256;
+        const unsigned blocks = (unsigned)((total + tb - 1) / tb);
+        if (!bc.is_velocity) {
+            k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi, g, bc, L::D, total, 1, 1);
+            launch_check("k_ghost_resolve");
+            return;
+        }
+        k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi, g, bc, L::D, total, 1, 0);
+        launch_check("k_ghost_resolve");
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (cfg.bc_type[s] != CLIP_BC_VELOCITY) continue;
+            const long long ps = plane_size(g, s);
+            k_ghost_velocity<CT><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(phi, g, vflag[s], s);
+            launch_check("k_ghost_velocity");
+        }
+        if (bc.mirror_gate) {
+            k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi, g, bc, L::D, total, 0, 1);
+            launch_check("k_ghost_resolve");
+        }
+    }
+
+    void init_fresh() override {
+        // Host phi over every node including ghosts (NsAllen.cu:768-824).
+        std::vector<CT> h(g.nodes);
+        for (int k = 0; k < g.ext[2]; ++k)
+            for (int j = 0; j < g.ext[1]; ++j)
+                for (int i = 0; i < g.ext[0]; ++i) h[g.idx(i, j, k)] = (CT)initial_phi(cfg, i, j, k);
+        HIP_CHECK(hipMemcpyAsync(phi, h.data(), h.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
+        // Distributions, p, u on all nodes of both buffers; outflow planes.
+        StepArgs<ST, CT> a = args(1, 1, 1);  // fout = buffer 0, uout = u[0], prev_out = prev[0]
+        const int tb = 256;
+        k_init<L, ST, CT, SP><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(
+            a, fbuf[1], gbuf[1], (CT)(cfg.reference_length / 2.0), (CT)cfg.sigma);
+        launch_check("k_init");
+        // Time-invariant inlet populations inside the disc, side by side.
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (cfg.bc_type[s] != CLIP_BC_VELOCITY) continue;
+            const long long ps = plane_size(g, s);
+            std::vector<CT> U(3 * ps);
+            const int ax = s / 2;
+            const int t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
+            for (long long t = 0; t < ps; ++t) {
+                int cc[3];
+                cc[t1] = (int)(t % g.ext[t1]);
+                cc[t2] = (int)(t / g.ext[t1]);
+                cc[ax] = (s & 1) ? g.dmax[ax] + 1 : g.dmin[ax] - 1;
+                double Uv[3];
+                inlet_velocity(cfg, s, cc, Uv);
+                for (int d = 0; d < 3; ++d) U[3 * t + d] = (CT)Uv[d];
+            }
+            CT* dU = scratch;
+            HIP_CHECK(hipMemcpyAsync(dU, U.data(), U.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
+            k_inlet_init<L, ST, CT, SP><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(
+                g, vflag[s], s, phi, fbuf[0], gbuf[0], fbuf[1], gbuf[1], dU);
+            launch_check("k_inlet_init");
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+        // First collision straight from the initial f_post = f (buffer 0).
+        StepArgs<ST, CT> c = args(0, 0, 1);  // pull 0 -> write 1; u from u[0]; prev_out = prev[0]
+        k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(c);
+        launch_check("k_fused");
+        HIP_CHECK(hipStreamSynchronize(stream));
+        cur = 1;
+        ucur = 0;
+        pcur = 0;
+        direct_old = true;
+        direct_f_interior.clear();
+        direct_g_interior.clear();
+        step_count = 0;
+        initialized = true;
+    }
+
+    void step(int n) override {
+        t_total = t_ghost = t_phase = t_fused = 0;
+        for (int s = 0; s < n; ++s) {
+            StepArgs<ST, CT> a = args(cur, ucur, pcur);
+            const bool tm = timing_on;
+            if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
+            ghost_refresh();
+            if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
+            launch_phase_pull(a);
+            if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
+            launch_fused_pull<MODE_FULL>(a);
+            if (tm) {
+                HIP_CHECK(hipEventRecord(ev[3], stream));
+                HIP_CHECK(hipEventSynchronize(ev[3]));
+                float m1, m2, m3;
+                HIP_CHECK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+                HIP_CHECK(hipEventElapsedTime(&m2, ev[1], ev[2]));
+                HIP_CHECK(hipEventElapsedTime(&m3, ev[2], ev[3]));
+                t_ghost += m1;
+                t_phase += m2;
+                t_fused += m3;
+                t_total += m1 + m2 + m3;
+            }
+            cur = 1 - cur;
+            ucur = 1 - ucur;
+            pcur = 1 - pcur;
+            direct_old = false;
+            step_count++;
+        }
+    }
+
+    void timing(double* total, double* gh, double* pha, double* fu) const override {
+        if (total) *total = t_total;
+        if (gh) *gh = t_ghost;
+        if (pha) *pha = t_phase;
+        if (fu) *fu = t_fused;
+    }
+    void enable_timing(bool on) override { timing_on = on; }
+
+    template <class T>
+    void download(const T* dev, size_t count, std::vector<double>& out) {
+        std::vector<T> h(count);
+        HIP_CHECK(hipMemcpyAsync(h.data(), dev, count * sizeof(T), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        out.resize(count);
+        for (size_t i = 0; i < count; ++i) out[i] = (double)h[i];
+    }
+
+    // Exports f_post/g_post (pull of the buffer the last step read, with the
+    // u and outflow planes that pull used) into `scratch`; with is_f also
+    // refreshes p = sum_q f_post.
+    void export_post(bool is_f) {
+        const int old = 1 - cur;
+        StepArgs<ST, CT> a = args(old, 1 - ucur, 1 - pcur);
+        const int tb = 256;
+        const unsigned blocks = (unsigned)((g.nodes + tb - 1) / tb);
+        if (direct_old)  // p is the stored one (initial or resumed)
+            k_export_post<L, ST, CT, SP, true><<<blocks, tb, 0, stream>>>(a, is_f, scratch, nullptr);
+        else
+            k_export_post<L, ST, CT, SP, false><<<blocks, tb, 0, stream>>>(a, is_f, scratch, is_f ? p : nullptr);
+        launch_check("k_export_post");
+    }
+
+    void get_field(int field, std::vector<double>& out) override {
+        const size_t nodes = (size_t)g.nodes;
+        const int tb = 256;
+        switch (field) {
+            case CLIP_FIELD_PHI: download(phi, nodes, out); return;
+            case CLIP_FIELD_RHO: {
+                download(phi, nodes, out);
+                for (auto& v : out) v = (double)(ph.rho_l + (CT)v * (ph.rho_h - ph.rho_l));
+                return;
+            }
+            case CLIP_FIELD_U: download(u[ucur], nodes * L::D, out); return;
+            case CLIP_FIELD_P: {
+                if (!direct_old) {
+                    export_post(true);
+                }
+                download(p, nodes, out);
+                return;
+            }
+            case CLIP_FIELD_MU:
+            case CLIP_FIELD_DPHI:
+            case CLIP_FIELD_NORMAL: {
+                CT* mu = scratch;
+                CT* dp = scratch + nodes;
+                CT* nr = scratch + nodes * (1 + L::D);
+                if (L::Q < 1 + 2 * L::D) throw ConfigError("scratch too small");
+                k_export_stencil<L, CT><<<(unsigned)((nodes + tb - 1) / tb), tb, 0, stream>>>(phi, g, ph, mu, dp, nr);
+                launch_check("k_export_stencil");
+                if (field == CLIP_FIELD_MU) download(mu, nodes, out);
+                else if (field == CLIP_FIELD_DPHI) download(dp, nodes * L::D, out);
+                else download(nr, nodes * L::D, out);
+                return;
+            }
+            case CLIP_FIELD_F:
+            case CLIP_FIELD_G: {
+                const int old = 1 - cur;
+                const ST* src = (field == CLIP_FIELD_F) ? fbuf[old] : gbuf[old];
+                const long long count = (long long)nodes * QL;
+                k_export_raw<ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(src, scratch, count, g.nodes);
+                launch_check("k_export_raw");
+                download(scratch, count, out);
+                const auto& keep = (field == CLIP_FIELD_F) ? direct_f_interior : direct_g_interior;
+                if (direct_old && !keep.empty()) out = keep;
+                if (!direct_old) fill_periodic_ghosts(out, QL);
+                return;
+            }
+            case CLIP_FIELD_F_POST:
+            case CLIP_FIELD_G_POST: {
+                export_post(field == CLIP_FIELD_F_POST);
+                download(scratch, nodes * QL, out);
+                return;
+            }
+        }
+        throw ConfigError("unknown field");
+    }
+
+    // Periodic ghost cells are never read by the pull (it wraps), so they are
+    // filled here the way kernelPeriodicBoundary would (Solver.cu:75-141).
+    void fill_periodic_ghosts(std::vector<double>& a, int ndof) {
+        for (int ax = 0; ax < L::D; ++ax) {
+            if (!bc.periodic[ax]) continue;
+            for (int k = 0; k < g.ext[2]; ++k)
+                for (int j = 0; j < g.ext[1]; ++j)
+                    for (int i = 0; i < g.ext[0]; ++i) {
+                        int c[3] = {i, j, k};
+                        if (c[ax] != 0 && c[ax] != g.ext[ax] - 1) continue;
+                        int s[3] = {i, j, k};
+                        s[ax] = (c[ax] == 0) ? g.dmax[ax] : g.dmin[ax];
+                        for (int q = 0; q < ndof; ++q)
+                            a[(size_t)q * g.nodes + g.idx(i, j, k)] = a[(size_t)q * g.nodes + g.idx(s[0], s[1], s[2])];
+                    }
+        }
+    }
+
+    template <class T>
+    void upload(T* dev, const std::vector<double>& in) {
+        std::vector<T> h(in.size());
+        for (size_t i = 0; i < in.size(); ++i) h[i] = (T)in[i];
+        HIP_CHECK(hipMemcpyAsync(dev, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void upload_pop(ST* dev, const std::vector<double>& in) {
+        const long long count = (long long)in.size();
+        upload(scratch, in);
+        const int tb = 256;
+        k_import_raw<ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(dev, scratch, count, g.nodes);
+        launch_check("k_import_raw");
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void set_field(int field, const std::vector<double>& in) override {
+        switch (field) {
+            case CLIP_FIELD_PHI: upload(phi, in); break;
+            case CLIP_FIELD_U: upload(u[ucur], in); break;
+            case CLIP_FIELD_P: upload(p, in); break;
+            default: throw ConfigError("field cannot be set directly; use a checkpoint resume");
+        }
+        // Recompute the pending collision from the modified state.
+        if (initialized) recollide();
+    }
+
+    // Re-runs the collision that produced pop[cur] from the state the last
+    // step left behind (used after a field was modified).
+    void recollide() {
+        const int old = 1 - cur;
+        if (!direct_old) export_post(true);  // p from the current f_post
+        // pull pop[old] with the u / outflow planes of that pull; collide with u[ucur]
+        StepArgs<ST, CT> a = args(old, 1 - ucur, 1 - pcur);
+        a.ucol = u[ucur];
+        a.uout = nullptr;
+        if (direct_old) {
+            k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
+            launch_check("k_fused");
+        } else {
+            launch_fused_pull<MODE_COLLIDE>(a);
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void resume(const std::vector<double>& f, const std::vector<double>& gg, const std::vector<double>& fpost,
+                const std::vector<double>& gpost, const std::vector<double>* phi_in,
+                const std::vector<double>* uold) override {
+        // Buffer 0: dev_f / dev_g with the interior replaced by f_post / g_post
+        // (read directly by the first kernels); buffer 1: dev_f / dev_g.
+        std::vector<double> a0 = f, b0 = gg;
+        for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
+            for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
+                for (int i = g.dmin[0]; i <= g.dmax[0]; ++i)
+                    for (int q = 0; q < QL; ++q) {
+                        const size_t t = (size_t)q * g.nodes + g.idx(i, j, k);
+                        a0[t] = fpost[t];
+                        b0[t] = gpost[t];
+                    }
+        upload_pop(fbuf[0], a0);
+        upload_pop(gbuf[0], b0);
+        upload_pop(fbuf[1], f);
+        upload_pop(gbuf[1], gg);
+        if (phi_in) upload(phi, *phi_in);
+        std::vector<double> zero((size_t)g.nodes * L::D, 0.0);
+        upload(u[0], uold ? *uold : zero);
+        // macroscopic() of the resume path (main.cu:67-71), then the collision.
+        StepArgs<ST, CT> a = args(0, 0, 1);  // pull 0 (direct) -> 1; u_old = u[0] -> u[1]; prev_out = prev[0]
+        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
+        launch_check("k_phase");
+        if (!phi_in) ghost_refresh();
+        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
+        launch_check("k_fused");
+        HIP_CHECK(hipStreamSynchronize(stream));
+        cur = 1;
+        ucur = 1;
+        pcur = 0;
+        direct_old = true;
+        direct_f_interior = f;
+        direct_g_interior = gg;
+        // p of the resumed macroscopic step
+        {
+            const int tb = 256;
+            StepArgs<ST, CT> e = args(0, 0, 1);
+            k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(e, true, scratch, p);
+            launch_check("k_export_post");
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+        initialized = true;
+    }
+
+    void uold_soa(std::vector<double>& out) override { download(u[1 - ucur], (size_t)g.nodes * L::D, out); }
+
+    double reduce(int field, unsigned* nonfinite) override {
+        const CT* src = phi;
+        if (field == CLIP_FIELD_RHO) {
+            // rho is a function of phi; sum phi and map
+            src = phi;
+        }
+        HIP_CHECK(hipMemsetAsync(red_bad, 0, sizeof(unsigned), stream));
+        const int blocks = 1024;
+        k_reduce_interior<CT><<<blocks, 256, 0, stream>>>(src, g, L::D, red_part, red_bad, 0.0);
+        launch_check("k_reduce_interior");
+        k_reduce_final<double><<<1, 256, 0, stream>>>(red_part, blocks, red_out);
+        launch_check("k_reduce_final");
+        double s = 0;
+        unsigned bad = 0;
+        HIP_CHECK(hipMemcpyAsync(&s, red_out, sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(&bad, red_bad, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (nonfinite) *nonfinite = bad;
+        if (field == CLIP_FIELD_RHO) {
+            const double ni = (double)(g.dmax[0] - g.dmin[0] + 1) * (g.dmax[1] - g.dmin[1] + 1) *
+                              (g.dmax[2] - g.dmin[2] + 1);
+            return cfg.rho_l * ni + s * (cfg.rho_h - cfg.rho_l);
+        }
+        return s;
+    }
+};
+
+}  // namespace clipamd

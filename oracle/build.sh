@@ -5,5 +5,5 @@
 set -e
 here=$(cd "$(dirname "$0")" && pwd)
 mkdir -p "$here/_build"
-g++ -O2 -std=c++17 -fopenmp -fPIC -shared -Wall -Wextra -Wno-unused-parameter \
+g++ -O2 -std=c++17 -ffp-contract=off -fopenmp -fPIC -shared -Wall -Wextra -Wno-unused-parameter \
     -I"$here/../include" "$here/clip_oracle.cpp" -o "$here/_build/libclip_oracle.so"

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -45,6 +46,11 @@ struct Lattice {
     int stress[5];
     double M[19][19];
     double Minv[19][19];
+    // M^-1 as integer numerators over one denominator per row (the form the
+    // reference's reconvert* uses, WMRT.cuh:77-92, 134-184), recovered from
+    // the numerical inverse.
+    double Minv_num[19][19];
+    double Minv_den[19];
     // in-directions of each side (pointing from the side into the domain),
     // axis-normal direction first: WMRT::wallBCMap (WMRT.cuh:214-237)
     int nin = 0;
@@ -154,6 +160,19 @@ void make_lattice(int D, Lattice& L) {
             for (int q = 0; q < 19; ++q) L.M[r][q] = moment3(r, L.e[q][0], L.e[q][1], L.e[q][2]);
     }
     invert(L.Q, L.M, L.Minv);
+    for (int q = 0; q < L.Q; ++q) {
+        int den = 1;
+        for (; den <= 1000; ++den) {
+            bool ok = true;
+            for (int r = 0; r < L.Q && ok; ++r) {
+                const double x = L.Minv[q][r] * den;
+                ok = std::fabs(x - std::nearbyint(x)) < 1e-9;
+            }
+            if (ok) break;
+        }
+        L.Minv_den[q] = den;
+        for (int r = 0; r < L.Q; ++r) L.Minv_num[q][r] = std::nearbyint(L.Minv[q][r] * den);
+    }
     for (int q = 0; q < L.Q; ++q)
         for (int p = 0; p < L.Q; ++p)
             if (L.e[p][0] == -L.e[q][0] && L.e[p][1] == -L.e[q][1] && L.e[p][2] == -L.e[q][2]) L.opp[q] = p;
@@ -259,8 +278,8 @@ void wmrt_filter(const Lattice& L, double* v, double s) {
     for (int k = 0; k < L.nstress; ++k) m[L.stress[k]] *= s;
     for (int q = 0; q < L.Q; ++q) {
         double acc = 0;
-        for (int r = 0; r < L.Q; ++r) acc += L.Minv[q][r] * m[r];
-        v[q] = acc;
+        for (int r = 0; r < L.Q; ++r) acc += L.Minv_num[q][r] * m[r];
+        v[q] = acc / L.Minv_den[q];
     }
 }
 
@@ -270,18 +289,26 @@ inline double tau_of(const clip_config& c, double phi) {
     return c.tau_l + phi * (c.tau_h - c.tau_l);
 }
 
-// calculateVF (NsAllen.cu:102-132): viscous force from the filtered gneq.
+// calculateVF (NsAllen.cu:102-132): viscous force from the filtered gneq,
+// with the reference's term order for each component.
 void viscous_force(const Oracle& o, const double* gneq, double tau, const double* dc, double* fv) {
-    double s[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int q = 0; q < o.Q; ++q)
-        for (int a = 0; a < o.D; ++a)
-            for (int b = 0; b < o.D; ++b) s[a][b] += o.L.e[q][a] * o.L.e[q][b] * gneq[q];
-    const double drho = o.c.rho_h - o.c.rho_l;
-    for (int a = 0; a < o.D; ++a) {
-        double acc = 0;
-        for (int b = 0; b < o.D; ++b) acc += s[a][b] * dc[b];
-        fv[a] = -tau * acc * drho;
+    double sxx = 0, sxy = 0, syy = 0, szy = 0, szx = 0, szz = 0;
+    for (int q = 0; q < o.Q; ++q) {
+        const int* e = o.L.e[q];
+        sxx += e[0] * e[0] * gneq[q];
+        sxy += e[0] * e[1] * gneq[q];
+        syy += e[1] * e[1] * gneq[q];
+        if (o.D == 3) {
+            szy += e[2] * e[1] * gneq[q];
+            szx += e[2] * e[0] * gneq[q];
+            szz += e[2] * e[2] * gneq[q];
+        }
     }
+    const double drho = o.c.rho_h - o.c.rho_l;
+    const double dz = (o.D == 3) ? dc[2] : 0.0;
+    fv[0] = -tau * (sxx * dc[0] + sxy * dc[1] + szx * dz) * drho;
+    fv[1] = -tau * (sxy * dc[0] + syy * dc[1] + szy * dz) * drho;
+    if (o.D == 3) fv[2] = -tau * (szz * dc[2] + szx * dc[0] + szy * dc[1]) * drho;
 }
 
 // Body force by case (NsAllen.cu:460-474).
@@ -896,6 +923,57 @@ int orc_collide(void* h) {
 int orc_macroscopic(void* h) {
     macroscopic(*(Oracle*)h);
     return 0;
+}
+
+// One named phase of the step, for the invariant tests:
+// "normals", "collide_g", "collide_f", "periodic", "velocity", "stream",
+// "wall", "mirror", "free_convect", "neumann", "macro_g", "chem_pot",
+// "gradient", "macro_f".
+int orc_phase(void* h, const char* name) {
+    Oracle& o = *(Oracle*)h;
+    const std::string n(name);
+    if (n == "normals") normals(o);
+    else if (n == "collide_g") collide_g(o);
+    else if (n == "collide_f") collide_f(o);
+    else if (n == "periodic") {
+        periodic_pass(o, o.f, o.Q);
+        periodic_pass(o, o.g, o.Q);
+        periodic_pass(o, o.phi, 1);
+    } else if (n == "velocity") velocity_pass(o);
+    else if (n == "stream") streaming(o);
+    else if (n == "wall") wall_pass(o);
+    else if (n == "mirror") mirror_pass(o);
+    else if (n == "free_convect") free_convect_pass(o);
+    else if (n == "neumann") neumann_pass(o);
+    else if (n == "macro_g") macro_g(o);
+    else if (n == "chem_pot") chemical_potential(o);
+    else if (n == "gradient") isotropic_gradient(o);
+    else if (n == "macro_f") macro_f(o);
+    else return -1;
+    return 0;
+}
+
+// f_prev / g_prev (outflow) of the oracle, same shape as F.
+int orc_get_prev(void* h, int which, double* out) {
+    Oracle& o = *(Oracle*)h;
+    const std::vector<double>& v = which ? o.gprev : o.fprev;
+    if (v.empty()) return -1;
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return 0;
+}
+int orc_set_prev(void* h, int which, const double* in) {
+    Oracle& o = *(Oracle*)h;
+    std::vector<double>& v = which ? o.gprev : o.fprev;
+    if (v.empty()) return -1;
+    std::memcpy(v.data(), in, v.size() * sizeof(double));
+    return 0;
+}
+
+// Velocity part of the equilibrium for one direction (Equilibrium_new).
+double orc_gamma(int dim, int q, const double* u) {
+    Lattice L;
+    make_lattice(dim, L);
+    return gamma_u(L, q, u);
 }
 
 long long orc_field_size(void* h, int field) {

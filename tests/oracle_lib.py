@@ -45,6 +45,14 @@ def load():
     lib.orc_filter.argtypes = [ctypes.c_int, D, ctypes.c_double]
     lib.orc_sdf.restype = ctypes.c_double
     lib.orc_sdf.argtypes = [ctypes.POINTER(_abi.ClipConfig), ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.orc_phase.restype = ctypes.c_int
+    lib.orc_phase.argtypes = [P, ctypes.c_char_p]
+    lib.orc_get_prev.restype = ctypes.c_int
+    lib.orc_get_prev.argtypes = [P, ctypes.c_int, D]
+    lib.orc_set_prev.restype = ctypes.c_int
+    lib.orc_set_prev.argtypes = [P, ctypes.c_int, D]
+    lib.orc_gamma.restype = ctypes.c_double
+    lib.orc_gamma.argtypes = [ctypes.c_int, ctypes.c_int, D]
     lib.orc_destroy.restype = None
     lib.orc_destroy.argtypes = [P]
     _lib = lib
@@ -84,6 +92,24 @@ class Oracle:
     def macroscopic(self):
         self.lib.orc_macroscopic(self.h)
 
+    def phase(self, *names):
+        for n in names:
+            if self.lib.orc_phase(self.h, n.encode()) != 0:
+                raise ValueError(n)
+        return self
+
+    def get_prev(self, which):
+        ex = self.config.extents()
+        out = np.empty(ex[0] * ex[1] * ex[2] * self.config.q)
+        if self.lib.orc_get_prev(self.h, which, _dptr(out)) != 0:
+            raise ValueError("no outflow side")
+        return out.reshape(ex[0], ex[1], ex[2], -1)
+
+    def set_prev(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if self.lib.orc_set_prev(self.h, which, _dptr(a)) != 0:
+            raise ValueError("no outflow side")
+
     def get(self, field):
         fid = _abi.FIELD[field]
         n = self.lib.orc_field_size(self.h, fid)
@@ -114,6 +140,12 @@ def lattice(dim):
     k = nin.value
     return {"M": M.reshape(Q, Q), "Minv": Minv.reshape(Q, Q), "e": e.reshape(Q, 3), "w": w,
             "in": inl[: 2 * dim * k].reshape(2 * dim, k)}
+
+
+def gamma(dim, q, u):
+    uu = np.zeros(3)
+    uu[: len(u)] = u
+    return load().orc_gamma(dim, q, _dptr(uu))
 
 
 def wmrt_filter(dim, v, s):
