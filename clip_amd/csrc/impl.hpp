@@ -34,9 +34,19 @@ struct Impl {
     virtual void step(int n) = 0;
     virtual void get_field(int field, std::vector<double>& out) = 0;  // SoA, ghosts included
     virtual void set_field(int field, const std::vector<double>& in) = 0;
+    // Resume from CLIP's file set.  With `phi`/`uold` (the extension file)
+    // the pulled buffer, outflow planes, u_old and the phi ghost ring are
+    // restored and the step kernels recompute the last macroscopic step
+    // exactly; without them the reference's restart is reproduced
+    // (f_post read directly, u_old = 0).
     virtual void resume(const std::vector<double>& f, const std::vector<double>& g, const std::vector<double>& fpost,
-                        const std::vector<double>& gpost, const std::vector<double>* phi,
+                        const std::vector<double>& gpost, const std::vector<double>* fprev,
+                        const std::vector<double>* gprev, const std::vector<double>* phi,
                         const std::vector<double>* uold) = 0;
+    // f_prev / g_prev as CLIP stores them: f_post of the previous step on the
+    // outflow planes (the entries the outflow condition reads), the current
+    // f_post elsewhere.  SoA, ghosts included.
+    virtual void prev_soa(bool is_f, std::vector<double>& out) = 0;
     virtual void uold_soa(std::vector<double>& out) = 0;
     virtual double reduce(int field, unsigned* nonfinite) = 0;
     virtual void sync() = 0;

@@ -49,7 +49,8 @@ struct DevGrid {
 
 struct DevBC {
     int type[6];
-    int periodic[3];
+    int periodic[3];    // axis wraps (either side periodic: Solver.cu:96, 110, 125)
+    int both_periodic[3];  // both sides periodic: no boundary-layer fix-ups on this axis
     int is_wall, is_fc, is_neumann, is_velocity, mirror_gate;
     const unsigned char* vflag[6];  // per velocity side: 1 = ghost node inside the inlet disc
 };
@@ -396,8 +397,8 @@ constexpr int BX = 64, BY = 4;
 // (inset = 0, used by the non-pulling kernels).
 template <class L>
 __device__ __forceinline__ bool node_of_thread(const DevGrid& g, const DevBC& bc, int inset, int c[3]) {
-    const int ix = inset && !bc.periodic[0], iy = inset && !bc.periodic[1];
-    const int iz = (L::D == 3) ? (inset && !bc.periodic[2]) : 0;
+    const int ix = inset && !bc.both_periodic[0], iy = inset && !bc.both_periodic[1];
+    const int iz = (L::D == 3) ? (inset && !bc.both_periodic[2]) : 0;
     c[0] = blockIdx.x * BX + threadIdx.x + g.dmin[0] + ix;
     c[1] = blockIdx.y * BY + threadIdx.y + g.dmin[1] + iy;
     c[2] = blockIdx.z + g.dmin[2] + iz;
@@ -411,7 +412,7 @@ __host__ __device__ inline ShellDesc make_shell(const DevGrid& g, const DevBC& b
     long long n[3];
     for (int a = 0; a < 3; ++a) n[a] = g.dmax[a] - g.dmin[a] + 1;
     for (int a = 0; a < L::D; ++a) {
-        if (bc.periodic[a]) continue;
+        if (bc.both_periodic[a]) continue;
         long long cnt = 2;
         for (int b = 0; b < 3; ++b) {
             if (b == a) continue;

@@ -354,12 +354,12 @@ extern "C" int clip_checkpoint_save(clip_solver* s, const char* dir) {
             write_field(d + "/" + pf.name + ".bin", aos, single);
         }
         if (fc) {
-            // f_prev is overwritten from f_post before its first use after a
-            // resume (NsAllen.cu:850-854); the post-streaming arrays are stored.
-            s->impl->get_field(CLIP_FIELD_G_POST, soa);
+            // f_post of the previous step on the outflow planes (what the
+            // outflow condition reads), the current f_post elsewhere.
+            s->impl->prev_soa(false, soa);
             soa_to_aos(soa, aos.data(), e, Q);
             write_field(d + "/dev_g_prev.bin", aos, single);
-            s->impl->get_field(CLIP_FIELD_F_POST, soa);
+            s->impl->prev_soa(true, soa);
             soa_to_aos(soa, aos.data(), e, Q);
             write_field(d + "/dev_f_prev.bin", aos, single);
         }
@@ -410,6 +410,15 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
         aos_to_soa(aos.data(), g, e, Q);
         read_field(d + "/dev_f.bin", aos, nodes * Q, single);
         aos_to_soa(aos.data(), f, e, Q);
+        std::vector<double> fprev, gprev;
+        bool have_prev = false;
+        if (fs::exists(d + "/dev_f_prev.bin") && fs::exists(d + "/dev_g_prev.bin")) {
+            read_field(d + "/dev_f_prev.bin", aos, nodes * Q, single);
+            aos_to_soa(aos.data(), fprev, e, Q);
+            read_field(d + "/dev_g_prev.bin", aos, nodes * Q, single);
+            aos_to_soa(aos.data(), gprev, e, Q);
+            have_prev = true;
+        }
         SimInfoFile ti;
         read_raw(d + "/timeInfo.bin", &ti, 1);
         std::vector<double> phi, uold;
@@ -431,7 +440,8 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
             if (!st) throw std::runtime_error("truncated clip_amd_state.bin");
             have_state = true;
         }
-        s->impl->resume(f, g, fpost, gpost, have_state ? &phi : nullptr, have_state ? &uold : nullptr);
+        s->impl->resume(f, g, fpost, gpost, have_prev ? &fprev : nullptr, have_prev ? &gprev : nullptr,
+                        have_state ? &phi : nullptr, have_state ? &uold : nullptr);
         s->impl->step_count = ti.currentStep;
     });
     if (rc == CLIP_ERR_ARG || rc == CLIP_ERR_IO) {

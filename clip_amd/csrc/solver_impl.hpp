@@ -89,8 +89,10 @@ struct SolverImpl : Impl {
             isNeu |= c.bc_type[s] == CLIP_BC_NEUMANN;
             isVel |= c.bc_type[s] == CLIP_BC_VELOCITY;
         }
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a) {
             bc.periodic[a] = (a < L::D) && (c.bc_type[2 * a] == CLIP_BC_PERIODIC || c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC);
+            bc.both_periodic[a] = (a < L::D) && (c.bc_type[2 * a] == CLIP_BC_PERIODIC && c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC);
+        }
         bc.is_wall = isWall;
         bc.is_fc = isFC;
         bc.is_neumann = isNeu;
@@ -127,6 +129,8 @@ struct SolverImpl : Impl {
                 for (int b = 0; b < 2; ++b) {
                     prevf[b][s] = alloc<CT>(plane_size(g, s) * QL);
                     prevg[b][s] = alloc<CT>(plane_size(g, s) * QL);
+                    HIP_CHECK(hipMemset(prevf[b][s], 0, plane_size(g, s) * QL * sizeof(CT)));
+                    HIP_CHECK(hipMemset(prevg[b][s], 0, plane_size(g, s) * QL * sizeof(CT)));
                 }
             if (c.bc_type[s] == CLIP_BC_VELOCITY) {
                 const long long ps = plane_size(g, s);
@@ -206,7 +210,7 @@ struct SolverImpl : Impl {
     dim3 grid_box(int inset) const {
         int n[3];
         for (int a = 0; a < 3; ++a) {
-            const int in = (a < L::D && inset && !bc.periodic[a]) ? 1 : 0;
+            const int in = (a < L::D && inset && !bc.both_periodic[a]) ? 1 : 0;
             n[a] = g.dmax[a] - g.dmin[a] + 1 - 2 * in;
         }
         if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return dim3(0, 0, 0);
@@ -498,11 +502,81 @@ struct SolverImpl : Impl {
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
+    // Visits the inward populations of every outflow boundary node:
+    // fn(side, q, node index, plane offset).
+    template <class Fn>
+    void for_outflow_entries(Fn fn) const {
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;
+            const int a = s / 2, inward = (s & 1) ? -1 : 1;
+            for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
+                for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
+                    for (int i = g.dmin[0]; i <= g.dmax[0]; ++i) {
+                        const int c[3] = {i, j, k};
+                        if (c[a] != ((s & 1) ? g.dmax[a] : g.dmin[a])) continue;
+                        for (int q = 0; q < QL; ++q)
+                            if (L::e(q, a) == inward)
+                                fn(s, q, g.idx(i, j, k), (long long)q * plane_size(g, s) + plane_index(g, s, c));
+                    }
+        }
+    }
+
+    void prev_soa(bool is_f, std::vector<double>& out) override {
+        get_field(is_f ? CLIP_FIELD_F_POST : CLIP_FIELD_G_POST, out);
+        if (direct_old) return;
+        const int set = 1 - pcur;  // the planes the last step's pull read
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;
+            std::vector<double> plane;
+            download(is_f ? prevf[set][s] : prevg[set][s], (size_t)plane_size(g, s) * QL, plane);
+            for_outflow_entries([&](int side, int q, long long node, long long off) {
+                if (side == s) out[(size_t)q * g.nodes + node] = plane[off];
+            });
+        }
+    }
+
     void resume(const std::vector<double>& f, const std::vector<double>& gg, const std::vector<double>& fpost,
-                const std::vector<double>& gpost, const std::vector<double>* phi_in,
-                const std::vector<double>* uold) override {
-        // Buffer 0: dev_f / dev_g with the interior replaced by f_post / g_post
-        // (read directly by the first kernels); buffer 1: dev_f / dev_g.
+                const std::vector<double>& gpost, const std::vector<double>* fprev, const std::vector<double>* gprev,
+                const std::vector<double>* phi_in, const std::vector<double>* uold) override {
+        const int tb = 256;
+        if (phi_in && uold) {
+            // Exact restart: pop[0] = dev_f / dev_g (pulled), outflow planes from
+            // dev_f_prev / dev_g_prev, u_old and phi with its ghost ring from the
+            // extension; then the same kernels as a step recompute macroscopic()
+            // and the pending collision.
+            upload_pop(fbuf[0], f);
+            upload_pop(gbuf[0], gg);
+            upload_pop(fbuf[1], f);
+            upload_pop(gbuf[1], gg);
+            upload(phi, *phi_in);
+            upload(u[0], *uold);
+            for (int s = 0; s < 2 * L::D; ++s) {
+                if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;
+                std::vector<double> pf((size_t)plane_size(g, s) * QL, 0.0), pg(pf.size(), 0.0);
+                for_outflow_entries([&](int side, int q, long long node, long long off) {
+                    if (side != s) return;
+                    pf[off] = fprev ? (*fprev)[(size_t)q * g.nodes + node] : fpost[(size_t)q * g.nodes + node];
+                    pg[off] = gprev ? (*gprev)[(size_t)q * g.nodes + node] : gpost[(size_t)q * g.nodes + node];
+                });
+                upload(prevf[0][s], pf);
+                upload(prevg[0][s], pg);
+            }
+            StepArgs<ST, CT> a = args(0, 0, 0);  // pull 0 -> 1; u_old = u[0] -> u[1]; prev 0 -> 1
+            launch_phase_pull(a);
+            launch_fused_pull<MODE_FULL>(a);
+            HIP_CHECK(hipStreamSynchronize(stream));
+            cur = 1;
+            ucur = 1;
+            pcur = 1;
+            direct_old = false;
+            direct_f_interior.clear();
+            direct_g_interior.clear();
+            initialized = true;
+            return;
+        }
+        // Reference restart (main.cu:67-71): buffer 0 holds dev_f / dev_g with
+        // the interior replaced by f_post / g_post (read directly), buffer 1
+        // dev_f / dev_g; macroscopic() with u_old = 0, then the collision.
         std::vector<double> a0 = f, b0 = gg;
         for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
             for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
@@ -516,16 +590,16 @@ struct SolverImpl : Impl {
         upload_pop(gbuf[0], b0);
         upload_pop(fbuf[1], f);
         upload_pop(gbuf[1], gg);
-        if (phi_in) upload(phi, *phi_in);
         std::vector<double> zero((size_t)g.nodes * L::D, 0.0);
-        upload(u[0], uold ? *uold : zero);
-        // macroscopic() of the resume path (main.cu:67-71), then the collision.
-        StepArgs<ST, CT> a = args(0, 0, 1);  // pull 0 (direct) -> 1; u_old = u[0] -> u[1]; prev_out = prev[0]
+        upload(u[0], zero);
+        StepArgs<ST, CT> a = args(0, 0, 1);  // direct pull of 0 -> 1; u_old = u[0] -> u[1]; prev_out = prev[0]
         k_phase<L, ST, CT, SP, true, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
         launch_check("k_phase");
-        if (!phi_in) ghost_refresh();
+        ghost_refresh();
         k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
         launch_check("k_fused");
+        k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(a, true, scratch, p);
+        launch_check("k_export_post");
         HIP_CHECK(hipStreamSynchronize(stream));
         cur = 1;
         ucur = 1;
@@ -533,14 +607,6 @@ struct SolverImpl : Impl {
         direct_old = true;
         direct_f_interior = f;
         direct_g_interior = gg;
-        // p of the resumed macroscopic step
-        {
-            const int tb = 256;
-            StepArgs<ST, CT> e = args(0, 0, 1);
-            k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(e, true, scratch, p);
-            launch_check("k_export_post");
-            HIP_CHECK(hipStreamSynchronize(stream));
-        }
         initialized = true;
     }
 
