@@ -41,8 +41,10 @@ struct Impl {
     // (f_post read directly, u_old = 0).
     virtual void resume(const std::vector<double>& f, const std::vector<double>& g, const std::vector<double>& fpost,
                         const std::vector<double>& gpost, const std::vector<double>* fprev,
-                        const std::vector<double>* gprev, const std::vector<double>* phi,
-                        const std::vector<double>* uold) = 0;
+                        const std::vector<double>* gprev, const std::vector<double>* phi_now,
+                        const std::vector<double>* phi_before, const std::vector<double>* uold) = 0;
+    // phi of the last macroscopic step and the one before, ghosts included.
+    virtual void phi_pair(std::vector<double>& now, std::vector<double>& before) = 0;
     // f_prev / g_prev as CLIP stores them: f_post of the previous step on the
     // outflow planes (the entries the outflow condition reads), the current
     // f_post elsewhere.  SoA, ghosts included.
@@ -65,5 +67,6 @@ struct Impl {
 // floating-point contraction so that it reproduces the oracle bit for bit).
 std::unique_ptr<Impl> make_impl_f64(const clip_config& c);
 std::unique_ptr<Impl> make_impl_f32(const clip_config& c);
+std::unique_ptr<Impl> make_impl_mixed(const clip_config& c, bool bf16);
 
 }  // namespace clipamd

@@ -3,17 +3,21 @@
 // Layout in HBM: structure of arrays, x fastest.  Node (i, j, k) of the
 // ghost-inclusive grid has index n = i + j*ext_x + k*ext_x*ext_y, population q
 // of buffer A sits at A[q*nodes + n].  f and g live in two ping-pong buffers
-// each (post-collision values, ghost cells included); phi has one buffer with
-// its ghost ring; u has two (old / new).
+// each (post-collision values, ghost cells included); phi in two (the phi of
+// the collision that wrote the buffer being pulled, and the new one, each with
+// its ghost ring); u in two (old / new).
 //
-// One time step is three launches:
-//   k_ghost_*   refresh the phi ghost ring from phi^n (periodic wrap, inlet,
-//               mirror) — the lagged ghost values the reference's stencils see
+// One time step is three kernel classes:
+//   k_ghost_*   refresh the new phi's ghost ring from the old phi (periodic
+//               wrap, inlet, mirror): the lagged ghost values the reference's
+//               stencils see
 //   k_phase     phi^{n+1} = sum_q g_post, g_post pulled from g* with every
 //               boundary fix-up resolved at pull time
 //   k_fused     pull f_post/g_post, phi stencil (mu, grad phi, normal),
 //               macroscopic p/u (from u_old), then both collisions; writes
 //               f*, g* into the other buffer and u into the other u buffer.
+// k_phase and k_fused run as a bulk launch (plain offset pulls with periodic
+// wrap) and a boundary-layer launch (nodes next to a non-periodic side).
 // Reference phases folded into these kernels (NSAllen::solve, NsAllen.cu:881-892):
 // normal_FD, kernelCollisionMRTg/f, kernelPeriodicBoundary, JetBoundary,
 // kernelStreaming, kernelHalfBounceBack, kernelFreeConvect, kernelNeumann,
@@ -21,9 +25,9 @@
 // Isotropic_Gradient, kernelMacroscopicf.
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
 
 #include "lattice.hpp"
 
@@ -45,11 +49,16 @@ struct DevGrid {
     long long sy, sz;  // strides of j and k (i has stride 1)
     long long nodes;
     __host__ __device__ long long idx(int i, int j, int k) const { return i + j * sy + k * sz; }
+    __host__ __device__ bool interior(const int c[3], int D) const {
+        bool in = c[0] >= dmin[0] && c[0] <= dmax[0] && c[1] >= dmin[1] && c[1] <= dmax[1];
+        if (D == 3) in = in && c[2] >= dmin[2] && c[2] <= dmax[2];
+        return in;
+    }
 };
 
 struct DevBC {
     int type[6];
-    int periodic[3];    // axis wraps (either side periodic: Solver.cu:96, 110, 125)
+    int periodic[3];       // axis wraps (either side periodic: Solver.cu:96, 110, 125)
     int both_periodic[3];  // both sides periodic: no boundary-layer fix-ups on this axis
     int is_wall, is_fc, is_neumann, is_velocity, mirror_gate;
     const unsigned char* vflag[6];  // per velocity side: 1 = ghost node inside the inlet disc
@@ -78,20 +87,40 @@ struct Phys {
 };
 
 // ------------------------------------------------------------ storage
-// Plain conversion between storage and arithmetic types.  The 16-bit
-// policies store populations as scaled deviations from w_q * ref; `ref`
-// is 0 for f and phi-at-rest for g (see StoreMixed).
+// Storage policies convert between the stored population and the arithmetic
+// type.  `ref` is the phase field of the node whose collision wrote the value
+// (0 for ghost cells and for f); the plain policies ignore it.
 template <class ST, class CT>
 struct StorePlain {
-    __device__ static CT ld(const ST* p, long long i, int /*q*/) { return (CT)p[i]; }
-    __device__ static void st(ST* p, long long i, int /*q*/, CT v) { p[i] = (ST)v; }
+    static constexpr bool uses_ref = false;
+    __device__ static CT ld(const ST* p, long long i, int /*q*/, CT /*ref*/) { return (CT)p[i]; }
+    __device__ static void st(ST* p, long long i, int /*q*/, CT v, CT /*ref*/) { p[i] = (ST)v; }
+};
+
+// 16-bit storage of FP32 populations as scaled deviations from the lattice-
+// weight equilibrium at rest: g is stored as (g - w_q phi_ref) * 2^10, f as
+// f * 2^10 (its rest equilibrium w_q p is ~0: p is the dynamic pressure).
+template <class L, class ST>
+struct StoreMixed {
+    static constexpr bool uses_ref = true;
+    static constexpr float kScale = 1024.0f;
+    __device__ static float to_f(__half h) { return __half2float(h); }
+    __device__ static float to_f(__hip_bfloat16 h) { return __bfloat162float(h); }
+    __device__ static void from_f(__half& h, float v) { h = __float2half_rn(v); }
+    __device__ static void from_f(__hip_bfloat16& h, float v) { h = __float2bfloat16(v); }
+    __device__ static float ld(const ST* p, long long i, int q, float ref) {
+        return to_f(p[i]) * (1.0f / kScale) + float(L::w(q)) * ref;
+    }
+    __device__ static void st(ST* p, long long i, int q, float v, float ref) {
+        from_f(p[i], (v - float(L::w(q)) * ref) * kScale);
+    }
 };
 
 // ------------------------------------------------------------ arguments
 // Boundary layer = interior nodes on a plane next to a non-periodic side.
-// The planes of the first non-periodic axis are taken whole, those of later
-// axes exclude the rows already covered.  The host fills `axes` (ascending)
-// and the node count of each axis' plane pair.
+// The planes of the first such axis are taken whole, those of later axes
+// exclude the rows already covered.  `axes` ascending, `count` = nodes of
+// each axis' plane pair.
 struct ShellDesc {
     int na;
     int axes[3];
@@ -108,7 +137,8 @@ struct StepArgs {
     const ST* gin;
     ST* fout;        // f* written by the collision
     ST* gout;
-    CT* phi;         // phi with ghost ring
+    CT* phi;            // new phi with ghost ring (k_phase writes, k_fused reads the stencil)
+    const CT* phi_ref;  // phi the pulled buffer was encoded with (nullptr = 0)
     const CT* uin;   // u of the previous macroscopic step (D planes); outflow fix-ups read it
     CT* uout;        // u of this macroscopic step
     const CT* ucol;  // u the collide-only mode collides with
@@ -119,6 +149,12 @@ struct StepArgs {
     CT* prev_g_out[6];
 };
 
+template <class SP, class CT>
+__device__ __forceinline__ CT gref(const CT* phi_ref, long long node) {
+    if constexpr (SP::uses_ref) return phi_ref ? phi_ref[node] : CT(0);
+    else return CT(0);
+}
+
 // ------------------------------------------------------------ pull with fix-ups
 // Post-streaming population q at interior node c, reproducing the reference's
 // streaming + boundary sequence:
@@ -126,13 +162,21 @@ struct StepArgs {
 //   lvl1  kernelHalfBounceBack (wall / slip wall, when any wall exists)
 //   lvl2  kernelFreeConvect (reads lvl1 of the inner neighbour, Jacobi)
 //   lvl3  kernelNeumann     (reads lvl2 of the inner neighbour)
-template <class L, class ST, class CT, class SP>
+// GREF: the distribution is stored relative to phi_ref (g of the mixed policy).
+template <class L, class ST, class CT, class SP, bool GREF>
 struct Puller {
     const ST* A;
     const DevGrid& g;
     const DevBC& bc;
     const CT* uold;
     const CT* const* prev;
+    const CT* phi_ref;
+
+    __device__ CT load(int q, const int s[3]) const {
+        const long long node = g.idx(s[0], s[1], s[2]);
+        const CT r = (GREF && g.interior(s, L::D)) ? gref<SP>(phi_ref, node) : CT(0);
+        return SP::ld(A, (long long)q * g.nodes + node, q, r);
+    }
 
     __device__ CT raw(const int c[3], int q) const {
         int s[3] = {c[0] - L::ex(q), c[1] - L::ey(q), c[2] - L::ez(q)};
@@ -156,12 +200,12 @@ struct Puller {
                     // boundary node next to the ghost (Solver.cu:621-633)
                     int b[3] = {s[0], s[1], s[2]};
                     b[a] = (side & 1) ? g.dmax[a] : g.dmin[a];
-                    return SP::ld(A, (long long)L::opp(q) * g.nodes + g.idx(b[0], b[1], b[2]), L::opp(q));
+                    return load(L::opp(q), b);
                 }
                 break;  // inside: the time-invariant inlet value stored in the ghost cell
             }
         }
-        return SP::ld(A, (long long)q * g.nodes + g.idx(s[0], s[1], s[2]), q);
+        return load(q, s);
     }
 
     __device__ CT lvl1(const int c[3], int q) const {
@@ -173,7 +217,7 @@ struct Puller {
                 const int t = bc.type[side];
                 if ((t == BC_WALL || t == BC_SLIP_WALL) && L::e(q, a) == inward &&
                     c[a] == ((side & 1) ? g.dmax[a] : g.dmin[a]))
-                    return SP::ld(A, (long long)L::opp(q) * g.nodes + g.idx(c[0], c[1], c[2]), L::opp(q));
+                    return load(L::opp(q), c);
             }
         }
         return raw(c, q);
@@ -251,7 +295,9 @@ __device__ __forceinline__ CT gamma_u(int q, const CT u[3]) {
     return CT(L::w(q)) * (CT(3) * eu + CT(4.5) * eu * eu - CT(1.5) * u2);
 }
 
-// gneq <- M^-1 S M gneq, stress moments scaled by s (NsAllen.cu:416-431).
+// gneq <- M^-1 S M gneq, stress moments scaled by s (NsAllen.cu:416-431),
+// with the reference's transform tables (integer rows, M^-1 rows over a
+// common denominator, true division in FP64).
 template <class L, class CT>
 __device__ __forceinline__ void wmrt_filter(CT v[L::Q], CT s) {
     CT m[L::Q];
@@ -305,6 +351,88 @@ __device__ __forceinline__ void viscous_force(const CT gneq[L::Q], CT tau, CT dr
     fv[2] = (L::D == 3) ? -tau * (szz * dc[2] + szx * dc[0] + szy * dc[1]) * drho : CT(0);
 }
 
+// ---- moment-space form of the filter (FP32 / mixed kernels) ----------------
+// filter(v) = v + (s-1) sum_k M^-1[:,k] m_k(v), k over the stress rows.  Its
+// second moments are Pi(v) + (s-1) sum_k m_k Pi(M^-1[:,k]); the forcing term
+// hlp_q ~ w_q e_q.F has zero stress moments (even rows, odd vector), so the
+// reference's two-pass collision reduces exactly to
+//   f*_q = p w_q + Gamma_q(u) + hlp_q/2 - (s-1) sum_k M^-1[q][k] m_k(gneq).
+template <class L>
+__host__ __device__ constexpr double minv_at(int q, int k) {
+    return double(L::minv_num(q, L::stress(k))) / double(L::minv_den(q));
+}
+template <class L>
+__host__ __device__ constexpr double pi_dual(int k, int a, int b) {
+    double acc = 0.0;
+    for (int q = 0; q < L::Q; ++q) acc += double(L::e(q, a) * L::e(q, b)) * minv_at<L>(q, k);
+    return acc;
+}
+
+// Stress moments m_k and second moments Pi_ab of v.
+template <class L, class CT>
+__device__ __forceinline__ void stress_moments(const CT v[L::Q], CT mk[L::NSTRESS], CT pi[3][3]) {
+#pragma unroll
+    for (int k = 0; k < L::NSTRESS; ++k) {
+        CT acc = CT(0);
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const int c = L::M(L::stress(k), q);
+            if (c == 1) acc += v[q];
+            else if (c == -1) acc -= v[q];
+            else if (c != 0) acc += CT(c) * v[q];
+        }
+        mk[k] = acc;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) pi[a][b] = CT(0);
+#pragma unroll
+    for (int a = 0; a < L::D; ++a)
+#pragma unroll
+        for (int b = a; b < L::D; ++b) {
+            CT acc = CT(0);
+#pragma unroll
+            for (int q = 0; q < L::Q; ++q) {
+                const int c = L::e(q, a) * L::e(q, b);
+                if (c == 1) acc += v[q];
+                else if (c == -1) acc -= v[q];
+            }
+            pi[a][b] = acc;
+            pi[b][a] = acc;
+        }
+}
+
+// Viscous force from the filtered second moments.
+template <class L, class CT>
+__device__ __forceinline__ void viscous_force_moments(const CT mk[L::NSTRESS], const CT pi[3][3], CT s, CT tau,
+                                                      CT drho, const CT dc[3], CT fv[3]) {
+    CT pf[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (a >= L::D || b >= L::D) {
+                pf[a][b] = CT(0);
+                continue;
+            }
+            CT acc = CT(0);
+#pragma unroll
+            for (int k = 0; k < L::NSTRESS; ++k) {
+                const double cf = pi_dual<L>(k, a, b);
+                if (cf != 0.0) acc += CT(cf) * mk[k];
+            }
+            pf[a][b] = pi[a][b] + (s - CT(1)) * acc;
+        }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        CT acc = CT(0);
+#pragma unroll
+        for (int b = 0; b < L::D; ++b) acc += pf[a][b] * dc[b];
+        fv[a] = (a < L::D) ? -tau * acc * drho : CT(0);
+    }
+}
+
 template <class CT>
 __device__ __forceinline__ CT tau_of(const Phys<CT>& ph, CT phi) {
     if (phi < CT(0)) return ph.tau_l;
@@ -328,13 +456,13 @@ __device__ __forceinline__ void body_force(const Phys<CT>& ph, CT rho, CT fg[3])
 }
 
 // phi stencil at interior node n: isotropic gradient (NsAllen.cu:280-322) and
-// Laplacian of the chemical potential (NsAllen.cu:207-236).
+// Laplacian of the chemical potential (NsAllen.cu:207-236); sums accumulate
+// in lexicographic (x offset, y offset, z offset) order.
 template <class L, class CT>
 __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const DevGrid& g, long long n, CT& c0,
                                             CT dc[3], CT& lap) {
     c0 = phi[n];
     if (L::D == 2) {
-        // sums in lexicographic (x offset, then y offset) order
         const long long sy = g.sy;
         const CT xp = phi[n + 1], xm = phi[n - 1], yp = phi[n + sy], ym = phi[n - sy];
         const CT pp = phi[n + 1 + sy], pm = phi[n + 1 - sy], mp = phi[n - 1 + sy], mm = phi[n - 1 - sy];
@@ -353,8 +481,6 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
             for (int b = 0; b < 3; ++b)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) v[a][b][d] = phi[n + (a - 1) + (b - 1) * sy + (d - 1) * sz];
-        // gradient: faces 4/9, edges 1/9, corners 1/36, all times 1/2; the
-        // +side and -side sums accumulate in lexicographic (x, y, z) order
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             CT plus[3] = {0, 0, 0}, minus[3] = {0, 0, 0};
@@ -389,12 +515,12 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
     }
 }
 
-// ------------------------------------------------------------ kernels
+// ------------------------------------------------------------ node selection
 constexpr int BX = 64, BY = 4;
 
 // Thread -> node of the bulk region: the interior box shrunk by one node
-// along every non-periodic lattice axis (inset = 1), or the whole interior
-// (inset = 0, used by the non-pulling kernels).
+// along every axis that is not periodic on both sides (inset = 1), or the
+// whole interior (inset = 0, used by the non-pulling kernels).
 template <class L>
 __device__ __forceinline__ bool node_of_thread(const DevGrid& g, const DevBC& bc, int inset, int c[3]) {
     const int ix = inset && !bc.both_periodic[0], iy = inset && !bc.both_periodic[1];
@@ -404,7 +530,6 @@ __device__ __forceinline__ bool node_of_thread(const DevGrid& g, const DevBC& bc
     c[2] = blockIdx.z + g.dmin[2] + iz;
     return c[0] <= g.dmax[0] - ix && c[1] <= g.dmax[1] - iy;
 }
-
 
 template <class L>
 __host__ __device__ inline ShellDesc make_shell(const DevGrid& g, const DevBC& bc) {
@@ -486,13 +611,14 @@ __device__ __forceinline__ long long pull_src(long long n, const PullOffsets& o,
 
 // Node selection of the step kernels: SHELL = boundary layer (resolved pulls),
 // !SHELL && !DIRECT = bulk (offset pulls, periodic wrap), DIRECT = every
-// interior node.
+// interior node (values read in place).
 template <class L, bool DIRECT, bool SHELL>
 __device__ __forceinline__ bool select_node(const DevGrid& g, const DevBC& bc, const ShellDesc& sd, int c[3]) {
     if (SHELL) return shell_node<L>(g, sd, c);
     return node_of_thread<L>(g, bc, DIRECT ? 0 : 1, c);
 }
 
+// ------------------------------------------------------------ step kernels
 // phi^{n+1} = sum_q g_post (kernelMacroscopicg, NsAllen.cu:516-540).
 template <class L, class ST, class CT, class SP, bool DIRECT, bool SHELL>
 __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
@@ -501,14 +627,18 @@ __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
     const long long n = a.g.idx(c[0], c[1], c[2]);
     CT sum = CT(0);
     if (DIRECT) {
+        const CT r = gref<SP>(a.phi_ref, n);
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q);
+        for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q, r);
     } else if (!SHELL) {
         const PullOffsets po = pull_offsets<L>(a.g, c);
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + pull_src<L>(n, po, q), q);
+        for (int q = 0; q < L::Q; ++q) {
+            const long long src = pull_src<L>(n, po, q);
+            sum += SP::ld(a.gin, (long long)q * a.g.nodes + src, q, gref<SP>(a.phi_ref, src));
+        }
     } else {
-        Puller<L, ST, CT, SP> P{a.gin, a.g, a.bc, a.uin, a.prev_g_in};
+        Puller<L, ST, CT, SP, true> P{a.gin, a.g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) sum += P.lvl3(c, q);
     }
@@ -519,9 +649,12 @@ enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 
 // Fused macroscopic + collision (kernelMacroscopicf NsAllen.cu:369-511,
 // normal_FD 241-275, kernelCollisionMRTg 545-584, kernelCollisionMRTf 589-763).
+// FP64: the reference's transforms and operation order.  FP32 / mixed: the
+// exact moment-space form of the same collision (see stress_moments).
 template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL>
 __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     constexpr int Q = L::Q;
+    constexpr bool MOMENT_FORM = sizeof(CT) == 4;
     int c[3];
     if (!select_node<L, DIRECT, SHELL>(a.g, a.bc, a.shell, c)) return;
     const DevGrid& g = a.g;
@@ -546,12 +679,12 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     // f_post
     CT fp[Q];
     {
-        Puller<L, ST, CT, SP> P{a.fin, g, a.bc, a.uin, a.prev_f_in};
+        Puller<L, ST, CT, SP, false> P{a.fin, g, a.bc, a.uin, a.prev_f_in, a.phi_ref};
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q);
+            if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q, CT(0));
             else if (!SHELL)
-                fp[q] = SP::ld(a.fin, (long long)q * g.nodes + pull_src<L>(n, po, q), q);
+                fp[q] = SP::ld(a.fin, (long long)q * g.nodes + pull_src<L>(n, po, q), q, CT(0));
             else
                 fp[q] = P.lvl3(c, q);
         }
@@ -567,18 +700,31 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     CT p, u[3];
     if (MODE == MODE_FULL) {
         // macroscopic p, u (kernelMacroscopicf): gneq from u_old
-        p = CT(0);
+        if constexpr (MOMENT_FORM) {
+            double pd = 0.0;  // pressure recovered in FP64
 #pragma unroll
-        for (int q = 0; q < Q; ++q) p += fp[q];
+            for (int q = 0; q < Q; ++q) pd += (double)fp[q];
+            p = (CT)pd;
+        } else {
+            p = CT(0);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) p += fp[q];
+        }
         CT uo[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) uo[d] = (d < L::D) ? a.uin[(long long)d * g.nodes + n] : CT(0);
         CT gneq[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) gneq[q] = fp[q] - (p * CT(L::w(q)) + gamma_u<L, CT>(q, uo));
-        wmrt_filter<L, CT>(gneq, s);
         CT fv[3];
-        viscous_force<L, CT>(gneq, tau, drho, dc, fv);
+        if constexpr (MOMENT_FORM) {
+            CT mk[L::NSTRESS], pi[3][3];
+            stress_moments<L, CT>(gneq, mk, pi);
+            viscous_force_moments<L, CT>(mk, pi, s, tau, drho, dc, fv);
+        } else {
+            wmrt_filter<L, CT>(gneq, s);
+            viscous_force<L, CT>(gneq, tau, drho, dc, fv);
+        }
         CT F[3], mom[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
@@ -613,25 +759,55 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     {
         const CT wc = CT(1) / (CT(0.5) + CT(3) * ph.mobility);
         const CT aterm = (CT(3) * ph.mobility * (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5))))) / ph.W;
-        Puller<L, ST, CT, SP> P{a.gin, g, a.bc, a.uin, a.prev_g_in};
+        Puller<L, ST, CT, SP, true> P{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
+        const CT rn = DIRECT ? gref<SP>(a.phi_ref, n) : CT(0);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             CT gp;
-            if (DIRECT) gp = SP::ld(a.gin, (long long)q * g.nodes + n, q);
-            else if (!SHELL)
-                gp = SP::ld(a.gin, (long long)q * g.nodes + pull_src<L>(n, po, q), q);
-            else
+            if (DIRECT) gp = SP::ld(a.gin, (long long)q * g.nodes + n, q, rn);
+            else if (!SHELL) {
+                const long long src = pull_src<L>(n, po, q);
+                gp = SP::ld(a.gin, (long long)q * g.nodes + src, q, gref<SP>(a.phi_ref, src));
+            } else
                 gp = P.lvl3(c, q);
             if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
             CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
             if (L::D == 3) en += CT(L::ez(q)) * nr[2];
             const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);
-            SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc);
+            SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc, c0);
         }
     }
 
-    // f collision (kernelCollisionMRTf): two filtered passes
-    {
+    // f collision (kernelCollisionMRTf)
+    if constexpr (MOMENT_FORM) {
+        CT ga[Q], gneq[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            ga[q] = gamma_u<L, CT>(q, u);
+            gneq[q] = fp[q] - (p * CT(L::w(q)) + ga[q]);
+        }
+        CT mk[L::NSTRESS], pi[3][3], fv[3];
+        stress_moments<L, CT>(gneq, mk, pi);
+        viscous_force_moments<L, CT>(mk, pi, s, tau, drho, dc, fv);
+        CT F[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
+        const CT h = CT(1.5) / rho;  // hlp_q / 2 = 1.5 w_q (e_q.F) / rho
+        const CT sm1 = s - CT(1);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            CT eF = CT(L::ex(q)) * F[0] + CT(L::ey(q)) * F[1];
+            if (L::D == 3) eF += CT(L::ez(q)) * F[2];
+            CT corr = CT(0);
+#pragma unroll
+            for (int k = 0; k < L::NSTRESS; ++k) {
+                const double cm = minv_at<L>(q, k);
+                if (cm != 0.0) corr += CT(cm) * mk[k];
+            }
+            const CT fs = p * CT(L::w(q)) + ga[q] + CT(L::w(q)) * h * eF - sm1 * corr;
+            SP::st(a.fout, (long long)q * g.nodes + n, q, fs, CT(0));
+        }
+    } else {
         CT ga[Q], gneq[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -654,7 +830,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         }
         wmrt_filter<L, CT>(gneq, s);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) SP::st(a.fout, (long long)q * g.nodes + n, q, fp[q] - gneq[q] + hlp[q]);
+        for (int q = 0; q < Q; ++q) SP::st(a.fout, (long long)q * g.nodes + n, q, fp[q] - gneq[q] + hlp[q], CT(0));
     }
 }
 
@@ -678,13 +854,23 @@ __device__ inline bool ghost_cell_of_thread(const DevGrid& g, int D, long long t
     return false;
 }
 
+// Reads phi at `src` for a ghost refresh: interior cells from the old phi,
+// ghost cells from the old phi (values of the previous step) or, in the
+// launches after the first, from the new phi (written earlier this step).
+template <class CT>
+__device__ inline CT ghost_source(const CT* oldp, const CT* newp, const DevGrid& g, int D, const int s[3],
+                                  int ghost_from_new) {
+    const long long i = g.idx(s[0], s[1], s[2]);
+    return (!g.interior(s, D) && ghost_from_new) ? newp[i] : oldp[i];
+}
+
 // Periodic ghost copy (Solver.cu:75-141) and/or mirror copy
 // (Solver.cu:927-978) of phi: ghost <- phi at the cell with every periodic
 // axis wrapped and every mirror axis clamped.  The source cell is never
 // written by the same launch.
 template <class CT>
-__global__ void k_ghost_resolve(CT* phi, DevGrid g, DevBC bc, int D, long long total, int do_periodic,
-                                int do_mirror) {
+__global__ void k_ghost_resolve(const CT* oldp, CT* newp, DevGrid g, DevBC bc, int D, long long total,
+                                int do_periodic, int do_mirror, int ghost_from_new) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int c[3];
@@ -696,9 +882,9 @@ __global__ void k_ghost_resolve(CT* phi, DevGrid g, DevBC bc, int D, long long t
         if (!lo && !hi) continue;
         // the mirror copy runs after the periodic copy, so on a mirror-type
         // side it decides the value (NsAllen.cu:884-887)
-        const int t = bc.type[2 * a + (hi ? 1 : 0)];
-        const bool mirror = bc.mirror_gate && (t == BC_WALL || t == BC_FREE_CONVECT || t == BC_NEUMANN ||
-                                               t == BC_SLIP_WALL || t == BC_DO_NOTHING);
+        const int ty = bc.type[2 * a + (hi ? 1 : 0)];
+        const bool mirror = bc.mirror_gate && (ty == BC_WALL || ty == BC_FREE_CONVECT || ty == BC_NEUMANN ||
+                                               ty == BC_SLIP_WALL || ty == BC_DO_NOTHING);
         if (do_mirror && mirror) {
             s[a] = lo ? g.dmin[a] : g.dmax[a];
             changed = true;
@@ -707,12 +893,12 @@ __global__ void k_ghost_resolve(CT* phi, DevGrid g, DevBC bc, int D, long long t
             changed = true;
         }
     }
-    if (changed) phi[g.idx(c[0], c[1], c[2])] = phi[g.idx(s[0], s[1], s[2])];
+    if (changed) newp[g.idx(c[0], c[1], c[2])] = ghost_source(oldp, newp, g, D, s, ghost_from_new);
 }
 
 // Inlet side, outside the disc: phi ghost <- boundary node (Solver.cu:621-633).
 template <class CT>
-__global__ void k_ghost_velocity(CT* phi, DevGrid g, const unsigned char* flag, int side) {
+__global__ void k_ghost_velocity(const CT* oldp, CT* newp, DevGrid g, int D, const unsigned char* flag, int side) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= plane_size(g, side)) return;
     const int a = side >> 1;
@@ -725,12 +911,13 @@ __global__ void k_ghost_velocity(CT* phi, DevGrid g, const unsigned char* flag, 
     if (flag[t]) return;
     int b[3] = {c[0], c[1], c[2]};
     b[a] = (side & 1) ? g.dmax[a] : g.dmin[a];
-    phi[g.idx(c[0], c[1], c[2])] = phi[g.idx(b[0], b[1], b[2])];
+    newp[g.idx(c[0], c[1], c[2])] = ghost_source(oldp, newp, g, D, b, 1);
 }
 
 // ------------------------------------------------------------ initialisation
 // KernelInitializeDistributions with the interior stencils of
-// deviceInitializer (NsAllen.cu:137-202, 897-908).  All nodes, both buffers.
+// deviceInitializer (NsAllen.cu:137-202, 897-908).  All nodes, both buffers;
+// interior g encoded relative to phi, ghost cells absolute.
 template <class L, class ST, class CT, class SP>
 __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -749,8 +936,7 @@ __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
     a.p[n] = p;
     for (int d = 0; d < L::D; ++d) a.uout[(long long)d * g.nodes + n] = CT(0);
     CT nr[3] = {0, 0, 0};
-    bool interior = true;
-    for (int d = 0; d < L::D; ++d) interior = interior && c[d] >= g.dmin[d] && c[d] <= g.dmax[d];
+    const bool interior = g.interior(c, L::D);
     if (interior) {
         CT cc, dc[3], lap;
         phi_stencil<L, CT>(a.phi, g, n, cc, dc, lap);
@@ -759,6 +945,7 @@ __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
         const CT t = sqrt(sq) + CT(1e-32);
         for (int d = 0; d < L::D; ++d) nr[d] = dc[d] / t;
     }
+    const CT gr = interior ? c0 : CT(0);
     const CT u0[3] = {0, 0, 0};
     const CT hw = (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5)))) / ph.W;
 #pragma unroll
@@ -770,10 +957,10 @@ __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
         const CT gv = c0 * (ga + CT(L::w(q))) - CT(0.5) * hlp;
         const CT fv = p * CT(L::w(q)) + ga;
         const long long i = (long long)q * g.nodes + n;
-        SP::st(a.fout, i, q, fv);
-        SP::st(f2, i, q, fv);
-        SP::st(a.gout, i, q, gv);
-        SP::st(g2, i, q, gv);
+        SP::st(a.fout, i, q, fv, CT(0));
+        SP::st(f2, i, q, fv, CT(0));
+        SP::st(a.gout, i, q, gv, gr);
+        SP::st(g2, i, q, gv, gr);
         if (a.bc.is_fc && interior) {
             store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fv);
             store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gv);
@@ -809,8 +996,8 @@ __global__ void k_inlet_init(DevGrid g, const unsigned char* flag, int side, con
         feq[q] = fa;
         geq[q] = cg * (fa + CT(L::w(q)));
         if (L::e(q, ax) == 0) {
-            const CT fq = SP::ld(fA, (long long)q * g.nodes + n, q);
-            const CT gq = SP::ld(gA, (long long)q * g.nodes + n, q);
+            const CT fq = SP::ld(fA, (long long)q * g.nodes + n, q, CT(0));
+            const CT gq = SP::ld(gA, (long long)q * g.nodes + n, q, CT(0));
             for (int d = 0; d < L::D; ++d)
                 if (d != ax) Mt[d] += CT(L::e(q, d)) * (fq - feq[q]);
             N += gq - geq[q];
@@ -829,16 +1016,16 @@ __global__ void k_inlet_init(DevGrid g, const unsigned char* flag, int side, con
                 corr += CT(L::e(q, d)) * Mt[d];
                 if (L::e(q, d) != 0) normal = false;
             }
-        const CT fb = SP::ld(fA, (long long)b * g.nodes + n, b);
-        const CT gb = SP::ld(gA, (long long)b * g.nodes + n, b);
+        const CT fb = SP::ld(fA, (long long)b * g.nodes + n, b, CT(0));
+        const CT gb = SP::ld(gA, (long long)b * g.nodes + n, b, CT(0));
         const CT fv = fb + feq[q] - feq[b] - (CT(1) / weight) * corr;
         CT gv = geq[q] - (gb - geq[b]);
         if (!normal) gv -= N / weight;
         const long long i = (long long)q * g.nodes + n;
-        SP::st(fA, i, q, fv);
-        SP::st(fB, i, q, fv);
-        SP::st(gA, i, q, gv);
-        SP::st(gB, i, q, gv);
+        SP::st(fA, i, q, fv, CT(0));
+        SP::st(fB, i, q, fv, CT(0));
+        SP::st(gA, i, q, gv, CT(0));
+        SP::st(gB, i, q, gv, CT(0));
     }
 }
 
@@ -856,33 +1043,43 @@ __global__ void k_export_post(StepArgs<ST, CT> a, bool is_f, CT* out, CT* pout) 
     c[0] = (int)(n % g.ext[0]);
     c[1] = (int)((n / g.ext[0]) % g.ext[1]);
     c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
-    bool interior = true;
-    for (int d = 0; d < L::D; ++d) interior = interior && c[d] >= g.dmin[d] && c[d] <= g.dmax[d];
-    Puller<L, ST, CT, SP> P{src, g, a.bc, a.uin, prev};
+    const bool interior = g.interior(c, L::D);
+    const CT* ref = is_f ? nullptr : a.phi_ref;
     CT sum = 0;
     for (int q = 0; q < L::Q; ++q) {
         CT v;
-        if (!interior || DIRECT) v = SP::ld(src, (long long)q * g.nodes + n, q);
-        else v = P.lvl3(c, q);
+        if (!interior) v = SP::ld(src, (long long)q * g.nodes + n, q, CT(0));
+        else if (DIRECT) v = SP::ld(src, (long long)q * g.nodes + n, q, gref<SP>(ref, n));
+        else if (is_f) v = Puller<L, ST, CT, SP, false>{src, g, a.bc, a.uin, prev, ref}.lvl3(c, q);
+        else v = Puller<L, ST, CT, SP, true>{src, g, a.bc, a.uin, prev, ref}.lvl3(c, q);
         out[(long long)q * g.nodes + n] = v;
         sum += v;
     }
     if (pout && interior) pout[n] = sum;
 }
 
-// Stored populations into a dense CT array.
-template <class ST, class CT, class SP>
-__global__ void k_export_raw(const ST* src, CT* out, long long count, long long nodes) {
+// Stored populations <-> dense CT array (ref = phi at interior nodes for g,
+// nullptr for f).
+template <int Q, class ST, class CT, class SP>
+__global__ void k_export_raw(const ST* src, CT* out, DevGrid g, int D, const CT* ref) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    out[t] = SP::ld(src, t, (int)(t / nodes));
+    if (t >= g.nodes * Q) return;
+    const long long n = t % g.nodes;
+    const int q = (int)(t / g.nodes);
+    const int c[3] = {(int)(n % g.ext[0]), (int)((n / g.ext[0]) % g.ext[1]),
+                      (int)(n / ((long long)g.ext[0] * g.ext[1]))};
+    out[t] = SP::ld(src, t, q, g.interior(c, D) ? gref<SP>(ref, n) : CT(0));
 }
 
-template <class ST, class CT, class SP>
-__global__ void k_import_raw(ST* dst, const CT* in, long long count, long long nodes) {
+template <int Q, class ST, class CT, class SP>
+__global__ void k_import_raw(ST* dst, const CT* in, DevGrid g, int D, const CT* ref) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    SP::st(dst, t, (int)(t / nodes), in[t]);
+    if (t >= g.nodes * Q) return;
+    const long long n = t % g.nodes;
+    const int q = (int)(t / g.nodes);
+    const int c[3] = {(int)(n % g.ext[0]), (int)((n / g.ext[0]) % g.ext[1]),
+                      (int)(n / ((long long)g.ext[0] * g.ext[1]))};
+    SP::st(dst, t, q, in[t], g.interior(c, D) ? gref<SP>(ref, n) : CT(0));
 }
 
 // mu, grad phi and normal at interior nodes (ghosts left at 0).
@@ -894,9 +1091,7 @@ __global__ void k_export_stencil(const CT* phi, DevGrid g, Phys<CT> ph, CT* mu, 
     c[0] = (int)(n % g.ext[0]);
     c[1] = (int)((n / g.ext[0]) % g.ext[1]);
     c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
-    bool interior = true;
-    for (int d = 0; d < L::D; ++d) interior = interior && c[d] >= g.dmin[d] && c[d] <= g.dmax[d];
-    if (!interior) {
+    if (!g.interior(c, L::D)) {
         mu[n] = 0;
         for (int d = 0; d < L::D; ++d) dphi[d * g.nodes + n] = normal[d * g.nodes + n] = 0;
         return;
@@ -918,8 +1113,7 @@ __global__ void k_export_stencil(const CT* phi, DevGrid g, Phys<CT> ph, CT* mu, 
 // (Reporter::sum, Reporter.cu:50-114): wave64 shuffles, then LDS across the
 // block's waves, one partial per block; a second launch folds the partials.
 template <class CT>
-__global__ __launch_bounds__(256) void k_reduce_interior(const CT* x, DevGrid g, int D, double* part,
-                                                         unsigned* nonfinite, double ref) {
+__global__ __launch_bounds__(256) void k_reduce_interior(const CT* x, DevGrid g, double* part, unsigned* nonfinite) {
     __shared__ double ws[4];
     __shared__ unsigned wn[4];
     const long long nx = g.dmax[0] - g.dmin[0] + 1;
@@ -935,7 +1129,7 @@ __global__ __launch_bounds__(256) void k_reduce_interior(const CT* x, DevGrid g,
         const int k = (int)(t / (nx * ny)) + g.dmin[2];
         const double v = (double)x[g.idx(i, j, k)];
         if (!isfinite(v)) bad++;
-        else acc += v - ref;
+        else acc += v;
     }
     for (int off = 32; off > 0; off >>= 1) {
         acc += __shfl_xor(acc, off, 64);

@@ -35,6 +35,8 @@ std::unique_ptr<Impl> make_impl(const clip_config& c, int precision) {
     switch (precision) {
         case CLIP_PREC_F64: return make_impl_f64(c);
         case CLIP_PREC_F32: return make_impl_f32(c);
+        case CLIP_PREC_MIXED_F16: return make_impl_mixed(c, false);
+        case CLIP_PREC_MIXED_BF16: return make_impl_mixed(c, true);
     }
     throw ConfigError("unsupported precision " + std::to_string(precision));
 }
@@ -297,7 +299,7 @@ struct SimInfoFile {
 };
 static_assert(sizeof(SimInfoFile) == 32, "simInfo layout");
 
-const char kStateMagic[8] = {'C', 'L', 'I', 'P', 'A', 'M', 'D', '1'};
+const char kStateMagic[8] = {'C', 'L', 'I', 'P', 'A', 'M', 'D', '2'};
 
 }  // namespace
 
@@ -369,15 +371,18 @@ extern "C" int clip_checkpoint_save(clip_solver* s, const char* dir) {
         write_raw(d + "/timeInfo.bin", &ti, 1);
         std::vector<double> N(s->cfg.N, s->cfg.N + 3);
         write_field(d + "/domainSize.bin", N, single);
-        // Extension (no reference counterpart): phi with its ghost ring and the
-        // u that the last macroscopic step started from, for a bit-exact resume.
+        // Extension (no reference counterpart): phi of the last and the previous
+        // macroscopic step with their ghost rings, and the u the last
+        // macroscopic step started from, for a bit-exact resume.
         std::ofstream st(d + "/clip_amd_state.bin", std::ios::binary);
         if (!st.is_open()) throw std::runtime_error("Failed to open " + d + "/clip_amd_state.bin");
         st.write(kStateMagic, 8);
         const long long hdr[4] = {D, Q, (long long)nodes, s->impl->step_count};
         st.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
-        s->impl->get_field(CLIP_FIELD_PHI, soa);
+        std::vector<double> before;
+        s->impl->phi_pair(soa, before);
         st.write(reinterpret_cast<const char*>(soa.data()), (std::streamsize)(soa.size() * sizeof(double)));
+        st.write(reinterpret_cast<const char*>(before.data()), (std::streamsize)(before.size() * sizeof(double)));
         s->impl->uold_soa(soa);
         st.write(reinterpret_cast<const char*>(soa.data()), (std::streamsize)(soa.size() * sizeof(double)));
         if (!st) throw std::runtime_error("write failed: clip_amd_state.bin");
@@ -421,7 +426,7 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
         }
         SimInfoFile ti;
         read_raw(d + "/timeInfo.bin", &ti, 1);
-        std::vector<double> phi, uold;
+        std::vector<double> phi, phi_before, uold;
         bool have_state = false;
         const std::string sp = d + "/clip_amd_state.bin";
         if (fs::exists(sp)) {
@@ -434,14 +439,17 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
                 hdr[2] != (long long)nodes)
                 throw std::runtime_error("clip_amd_state.bin does not match this configuration");
             phi.resize(nodes);
+            phi_before.resize(nodes);
             uold.resize(nodes * D);
             st.read(reinterpret_cast<char*>(phi.data()), (std::streamsize)(phi.size() * sizeof(double)));
+            st.read(reinterpret_cast<char*>(phi_before.data()), (std::streamsize)(phi_before.size() * sizeof(double)));
             st.read(reinterpret_cast<char*>(uold.data()), (std::streamsize)(uold.size() * sizeof(double)));
             if (!st) throw std::runtime_error("truncated clip_amd_state.bin");
             have_state = true;
         }
         s->impl->resume(f, g, fpost, gpost, have_prev ? &fprev : nullptr, have_prev ? &gprev : nullptr,
-                        have_state ? &phi : nullptr, have_state ? &uold : nullptr);
+                        have_state ? &phi : nullptr, have_state ? &phi_before : nullptr,
+                        have_state ? &uold : nullptr);
         s->impl->step_count = ti.currentStep;
     });
     if (rc == CLIP_ERR_ARG || rc == CLIP_ERR_IO) {

@@ -2,13 +2,16 @@
 // device buffers of one handle and the launch sequence of every entry point.
 //
 // State of a handle between calls (after clip_init_fresh or clip_step):
-//   pop[cur]   f*, g* of the pending collision (the next step pulls it)
-//   pop[1-cur] f*, g* the last step pulled (the reference's dev_f / dev_g);
-//              right after initialisation or a resume it holds f_post directly
-//   u[ucur]    u of the last macroscopic step; u[1-ucur] the one before
-//   prev[pcur] outflow planes read by the next pull
-//   phi        phi with its ghost ring as the last step's stencils saw it
-// so one step is: ghost refresh, k_phase, k_fused (FULL), swap.
+//   pop[cur]       f*, g* of the pending collision (the next step pulls it)
+//   pop[1-cur]     f*, g* the last step pulled (the reference's dev_f / dev_g);
+//                  right after initialisation or a resume it holds f_post directly
+//   phib[ph]       phi of the last macroscopic step with the ghost ring its
+//                  stencils saw; pop[cur] is encoded relative to it
+//   phib[1-ph]     the phi before (pop[1-cur] is encoded relative to it)
+//   u[ucur]        u of the last macroscopic step; u[1-ucur] the one before
+//   prev[pcur]     outflow planes read by the next pull
+// One step: ghost refresh (phib[ph] -> ghosts of phib[1-ph]), k_phase,
+// k_fused (FULL), then every index flips.
 #pragma once
 
 #include "geometry.hpp"
@@ -32,7 +35,7 @@ struct SolverImpl : Impl {
     bool own_stream = false;
     ST* fbuf[2] = {nullptr, nullptr};
     ST* gbuf[2] = {nullptr, nullptr};
-    CT* phi = nullptr;
+    CT* phib[2] = {nullptr, nullptr};
     CT* u[2] = {nullptr, nullptr};
     CT* p = nullptr;
     CT* prevf[2][6] = {};
@@ -42,7 +45,8 @@ struct SolverImpl : Impl {
     double* red_part = nullptr;   // reduction partials
     unsigned* red_bad = nullptr;
     double* red_out = nullptr;
-    int cur = 1, ucur = 0, pcur = 0;
+    int cur = 1, ucur = 0, pcur = 0, phc = 0;
+    bool old_ref_zero = false;  // pop[1-cur] encoded without phi reference (reference restart)
     bool timing_on = false;
     hipEvent_t ev[8] = {};
     double t_total = 0, t_ghost = 0, t_phase = 0, t_fused = 0;
@@ -77,6 +81,8 @@ struct SolverImpl : Impl {
                 g.dmax[a] = g.ext[a] - 2;
             }
         }
+        for (int a = 0; a < L::D; ++a)
+            if (g.dmax[a] - g.dmin[a] + 1 < 3) throw ConfigError("the interior needs at least 3 nodes per axis");
         g.sy = g.ext[0];
         g.sz = (long long)g.ext[0] * g.ext[1];
         g.nodes = g.sz * g.ext[2];
@@ -90,8 +96,10 @@ struct SolverImpl : Impl {
             isVel |= c.bc_type[s] == CLIP_BC_VELOCITY;
         }
         for (int a = 0; a < 3; ++a) {
-            bc.periodic[a] = (a < L::D) && (c.bc_type[2 * a] == CLIP_BC_PERIODIC || c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC);
-            bc.both_periodic[a] = (a < L::D) && (c.bc_type[2 * a] == CLIP_BC_PERIODIC && c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC);
+            const bool lo = a < L::D && c.bc_type[2 * a] == CLIP_BC_PERIODIC;
+            const bool hi = a < L::D && c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC;
+            bc.periodic[a] = lo || hi;
+            bc.both_periodic[a] = lo && hi;
         }
         bc.is_wall = isWall;
         bc.is_fc = isFC;
@@ -117,8 +125,8 @@ struct SolverImpl : Impl {
             fbuf[b] = alloc<ST>(nodes * QL);
             gbuf[b] = alloc<ST>(nodes * QL);
             u[b] = alloc<CT>(nodes * L::D);
+            phib[b] = alloc<CT>(nodes);
         }
-        phi = alloc<CT>(nodes);
         p = alloc<CT>(nodes);
         scratch = alloc<CT>(nodes * std::max(QL, L::D));
         red_part = alloc<double>(1024);
@@ -151,9 +159,11 @@ struct SolverImpl : Impl {
             bc.vflag[s] = vflag[s];
         }
         for (int s = 2 * L::D; s < 6; ++s) bc.vflag[s] = nullptr;
-        HIP_CHECK(hipMemsetAsync(phi, 0, nodes * sizeof(CT), stream));
+        for (int b = 0; b < 2; ++b) {
+            HIP_CHECK(hipMemsetAsync(phib[b], 0, nodes * sizeof(CT), stream));
+            HIP_CHECK(hipMemsetAsync(u[b], 0, nodes * L::D * sizeof(CT), stream));
+        }
         HIP_CHECK(hipMemsetAsync(p, 0, nodes * sizeof(CT), stream));
-        for (int b = 0; b < 2; ++b) HIP_CHECK(hipMemsetAsync(u[b], 0, nodes * L::D * sizeof(CT), stream));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -181,7 +191,10 @@ struct SolverImpl : Impl {
 
     void sync() override { HIP_CHECK(hipStreamSynchronize(stream)); }
 
-    StepArgs<ST, CT> args(int pull_buf, int uin, int prev_in) {
+    // Arguments of a pass that pulls pop[pull_buf] (encoded relative to
+    // phi_ref) into pop[1-pull_buf], u[uin] -> u[1-uin], outflow planes
+    // prev[prev_in] -> prev[1-prev_in], stencils from `phi_new`.
+    StepArgs<ST, CT> args(int pull_buf, int uin, int prev_in, CT* phi_new, const CT* phi_ref) {
         StepArgs<ST, CT> a;
         a.g = g;
         a.bc = bc;
@@ -191,7 +204,8 @@ struct SolverImpl : Impl {
         a.gin = gbuf[pull_buf];
         a.fout = fbuf[1 - pull_buf];
         a.gout = gbuf[1 - pull_buf];
-        a.phi = phi;
+        a.phi = phi_new;
+        a.phi_ref = phi_ref;
         a.uin = u[uin];
         a.uout = u[1 - uin];
         a.ucol = u[uin];
@@ -205,8 +219,8 @@ struct SolverImpl : Impl {
         return a;
     }
 
-    // Grid over the interior box shrunk by `inset` along each non-periodic
-    // lattice axis (the bulk region of the pulling kernels when inset = 1).
+    // Grid over the interior box shrunk by `inset` along each axis that is
+    // not periodic on both sides (the bulk region of the pulling kernels).
     dim3 grid_box(int inset) const {
         int n[3];
         for (int a = 0; a < 3; ++a) {
@@ -225,7 +239,12 @@ struct SolverImpl : Impl {
     }
     unsigned shell_blocks() const { return (unsigned)((shell_nodes() + BX * BY - 1) / (BX * BY)); }
 
-    // Bulk (inner box) and boundary-layer launches of a pulling step kernel.
+    void launch_check(const char* what) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw HipError(std::string("launch of ") + what + ": " + hipGetErrorString(e));
+    }
+
+    // Bulk and boundary-layer launches of the pulling step kernels.
     template <int MODE>
     void launch_fused_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
@@ -240,31 +259,28 @@ struct SolverImpl : Impl {
         launch_check("k_phase");
     }
 
-    void launch_check(const char* what) {
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) throw HipError(std::string("launch of ") + what + ": " + hipGetErrorString(e));
-    }
-
-    void ghost_refresh() {
+    // Ghost ring of phi_new from phi_old (Solver.cu:75-141, 550-921, 927-978).
+    void ghost_refresh(const CT* phi_old, CT* phi_new) {
         long long total = 0;
         for (int s = 0; s < 2 * L::D; ++s) total += plane_size(g, s);
         const int tb = 256;
         const unsigned blocks = (unsigned)((total + tb - 1) / tb);
         if (!bc.is_velocity) {
-            k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi, g, bc, L::D, total, 1, 1);
+            k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi_old, phi_new, g, bc, L::D, total, 1, 1, 0);
             launch_check("k_ghost_resolve");
             return;
         }
-        k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi, g, bc, L::D, total, 1, 0);
+        k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi_old, phi_new, g, bc, L::D, total, 1, 0, 0);
         launch_check("k_ghost_resolve");
         for (int s = 0; s < 2 * L::D; ++s) {
             if (cfg.bc_type[s] != CLIP_BC_VELOCITY) continue;
             const long long ps = plane_size(g, s);
-            k_ghost_velocity<CT><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(phi, g, vflag[s], s);
+            k_ghost_velocity<CT><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(phi_old, phi_new, g, L::D,
+                                                                                 vflag[s], s);
             launch_check("k_ghost_velocity");
         }
         if (bc.mirror_gate) {
-            k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi, g, bc, L::D, total, 0, 1);
+            k_ghost_resolve<CT><<<blocks, tb, 0, stream>>>(phi_old, phi_new, g, bc, L::D, total, 0, 1, 1);
             launch_check("k_ghost_resolve");
         }
     }
@@ -275,9 +291,10 @@ struct SolverImpl : Impl {
         for (int k = 0; k < g.ext[2]; ++k)
             for (int j = 0; j < g.ext[1]; ++j)
                 for (int i = 0; i < g.ext[0]; ++i) h[g.idx(i, j, k)] = (CT)initial_phi(cfg, i, j, k);
-        HIP_CHECK(hipMemcpyAsync(phi, h.data(), h.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
+        for (int b = 0; b < 2; ++b)
+            HIP_CHECK(hipMemcpyAsync(phib[b], h.data(), h.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
         // Distributions, p, u on all nodes of both buffers; outflow planes.
-        StepArgs<ST, CT> a = args(1, 1, 1);  // fout = buffer 0, uout = u[0], prev_out = prev[0]
+        StepArgs<ST, CT> a = args(1, 1, 1, phib[0], phib[0]);  // fout = pop 0, uout = u[0], prev_out = prev[0]
         const int tb = 256;
         k_init<L, ST, CT, SP><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(
             a, fbuf[1], gbuf[1], (CT)(cfg.reference_length / 2.0), (CT)cfg.sigma);
@@ -301,19 +318,21 @@ struct SolverImpl : Impl {
             CT* dU = scratch;
             HIP_CHECK(hipMemcpyAsync(dU, U.data(), U.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
             k_inlet_init<L, ST, CT, SP><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(
-                g, vflag[s], s, phi, fbuf[0], gbuf[0], fbuf[1], gbuf[1], dU);
+                g, vflag[s], s, phib[0], fbuf[0], gbuf[0], fbuf[1], gbuf[1], dU);
             launch_check("k_inlet_init");
             HIP_CHECK(hipStreamSynchronize(stream));
         }
-        // First collision straight from the initial f_post = f (buffer 0).
-        StepArgs<ST, CT> c = args(0, 0, 1);  // pull 0 -> write 1; u from u[0]; prev_out = prev[0]
+        // First collision straight from the initial f_post = f (pop 0).
+        StepArgs<ST, CT> c = args(0, 0, 1, phib[0], phib[0]);
         k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(c);
         launch_check("k_fused");
         HIP_CHECK(hipStreamSynchronize(stream));
         cur = 1;
         ucur = 0;
         pcur = 0;
+        phc = 0;
         direct_old = true;
+        old_ref_zero = false;
         direct_f_interior.clear();
         direct_g_interior.clear();
         step_count = 0;
@@ -323,10 +342,12 @@ struct SolverImpl : Impl {
     void step(int n) override {
         t_total = t_ghost = t_phase = t_fused = 0;
         for (int s = 0; s < n; ++s) {
-            StepArgs<ST, CT> a = args(cur, ucur, pcur);
+            CT* phi_old = phib[phc];
+            CT* phi_new = phib[1 - phc];
+            StepArgs<ST, CT> a = args(cur, ucur, pcur, phi_new, phi_old);
             const bool tm = timing_on;
             if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
-            ghost_refresh();
+            ghost_refresh(phi_old, phi_new);
             if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
             launch_phase_pull(a);
             if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
@@ -346,7 +367,9 @@ struct SolverImpl : Impl {
             cur = 1 - cur;
             ucur = 1 - ucur;
             pcur = 1 - pcur;
+            phc = 1 - phc;
             direct_old = false;
+            old_ref_zero = false;
             step_count++;
         }
     }
@@ -368,12 +391,15 @@ struct SolverImpl : Impl {
         for (size_t i = 0; i < count; ++i) out[i] = (double)h[i];
     }
 
+    // phi the pulled buffer pop[1-cur] is encoded with (nullptr = 0).
+    const CT* old_ref() const { return old_ref_zero ? nullptr : phib[1 - phc]; }
+
     // Exports f_post/g_post (pull of the buffer the last step read, with the
     // u and outflow planes that pull used) into `scratch`; with is_f also
     // refreshes p = sum_q f_post.
     void export_post(bool is_f) {
         const int old = 1 - cur;
-        StepArgs<ST, CT> a = args(old, 1 - ucur, 1 - pcur);
+        StepArgs<ST, CT> a = args(old, 1 - ucur, 1 - pcur, phib[phc], old_ref());
         const int tb = 256;
         const unsigned blocks = (unsigned)((g.nodes + tb - 1) / tb);
         if (direct_old)  // p is the stored one (initial or resumed)
@@ -386,6 +412,7 @@ struct SolverImpl : Impl {
     void get_field(int field, std::vector<double>& out) override {
         const size_t nodes = (size_t)g.nodes;
         const int tb = 256;
+        CT* phi = phib[phc];
         switch (field) {
             case CLIP_FIELD_PHI: download(phi, nodes, out); return;
             case CLIP_FIELD_RHO: {
@@ -395,9 +422,7 @@ struct SolverImpl : Impl {
             }
             case CLIP_FIELD_U: download(u[ucur], nodes * L::D, out); return;
             case CLIP_FIELD_P: {
-                if (!direct_old) {
-                    export_post(true);
-                }
+                if (!direct_old) export_post(true);
                 download(p, nodes, out);
                 return;
             }
@@ -407,7 +432,6 @@ struct SolverImpl : Impl {
                 CT* mu = scratch;
                 CT* dp = scratch + nodes;
                 CT* nr = scratch + nodes * (1 + L::D);
-                if (L::Q < 1 + 2 * L::D) throw ConfigError("scratch too small");
                 k_export_stencil<L, CT><<<(unsigned)((nodes + tb - 1) / tb), tb, 0, stream>>>(phi, g, ph, mu, dp, nr);
                 launch_check("k_export_stencil");
                 if (field == CLIP_FIELD_MU) download(mu, nodes, out);
@@ -418,12 +442,14 @@ struct SolverImpl : Impl {
             case CLIP_FIELD_F:
             case CLIP_FIELD_G: {
                 const int old = 1 - cur;
-                const ST* src = (field == CLIP_FIELD_F) ? fbuf[old] : gbuf[old];
+                const bool is_f = field == CLIP_FIELD_F;
+                const ST* src = is_f ? fbuf[old] : gbuf[old];
                 const long long count = (long long)nodes * QL;
-                k_export_raw<ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(src, scratch, count, g.nodes);
+                k_export_raw<QL, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(
+                    src, scratch, g, L::D, is_f ? nullptr : old_ref());
                 launch_check("k_export_raw");
                 download(scratch, count, out);
-                const auto& keep = (field == CLIP_FIELD_F) ? direct_f_interior : direct_g_interior;
+                const auto& keep = is_f ? direct_f_interior : direct_g_interior;
                 if (direct_old && !keep.empty()) out = keep;
                 if (!direct_old) fill_periodic_ghosts(out, QL);
                 return;
@@ -464,18 +490,20 @@ struct SolverImpl : Impl {
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
-    void upload_pop(ST* dev, const std::vector<double>& in) {
+    // Encodes populations (dense, SoA) into a population buffer; `ref` = phi
+    // at interior nodes for g (nullptr: absolute).
+    void upload_pop(ST* dev, const std::vector<double>& in, const CT* ref) {
         const long long count = (long long)in.size();
         upload(scratch, in);
         const int tb = 256;
-        k_import_raw<ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(dev, scratch, count, g.nodes);
+        k_import_raw<QL, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(dev, scratch, g, L::D, ref);
         launch_check("k_import_raw");
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
     void set_field(int field, const std::vector<double>& in) override {
         switch (field) {
-            case CLIP_FIELD_PHI: upload(phi, in); break;
+            case CLIP_FIELD_PHI: upload(phib[phc], in); break;
             case CLIP_FIELD_U: upload(u[ucur], in); break;
             case CLIP_FIELD_P: upload(p, in); break;
             default: throw ConfigError("field cannot be set directly; use a checkpoint resume");
@@ -490,7 +518,7 @@ struct SolverImpl : Impl {
         const int old = 1 - cur;
         if (!direct_old) export_post(true);  // p from the current f_post
         // pull pop[old] with the u / outflow planes of that pull; collide with u[ucur]
-        StepArgs<ST, CT> a = args(old, 1 - ucur, 1 - pcur);
+        StepArgs<ST, CT> a = args(old, 1 - ucur, 1 - pcur, phib[phc], old_ref());
         a.ucol = u[ucur];
         a.uout = nullptr;
         if (direct_old) {
@@ -535,20 +563,27 @@ struct SolverImpl : Impl {
         }
     }
 
+    void phi_pair(std::vector<double>& now, std::vector<double>& before) override {
+        download(phib[phc], (size_t)g.nodes, now);
+        download(phib[1 - phc], (size_t)g.nodes, before);
+    }
+
     void resume(const std::vector<double>& f, const std::vector<double>& gg, const std::vector<double>& fpost,
                 const std::vector<double>& gpost, const std::vector<double>* fprev, const std::vector<double>* gprev,
-                const std::vector<double>* phi_in, const std::vector<double>* uold) override {
+                const std::vector<double>* phi_now, const std::vector<double>* phi_before,
+                const std::vector<double>* uold) override {
         const int tb = 256;
-        if (phi_in && uold) {
-            // Exact restart: pop[0] = dev_f / dev_g (pulled), outflow planes from
-            // dev_f_prev / dev_g_prev, u_old and phi with its ghost ring from the
-            // extension; then the same kernels as a step recompute macroscopic()
-            // and the pending collision.
-            upload_pop(fbuf[0], f);
-            upload_pop(gbuf[0], gg);
-            upload_pop(fbuf[1], f);
-            upload_pop(gbuf[1], gg);
-            upload(phi, *phi_in);
+        if (phi_now && phi_before && uold) {
+            // Exact restart: pop 0 = dev_f / dev_g (pulled) encoded relative to
+            // the phi that wrote them, outflow planes from dev_f_prev /
+            // dev_g_prev, u_old and both phi fields from the extension; then the
+            // step kernels recompute macroscopic() and the pending collision.
+            upload(phib[0], *phi_before);
+            upload(phib[1], *phi_now);
+            upload_pop(fbuf[0], f, nullptr);
+            upload_pop(gbuf[0], gg, phib[0]);
+            upload_pop(fbuf[1], f, nullptr);
+            upload_pop(gbuf[1], gg, phib[0]);
             upload(u[0], *uold);
             for (int s = 0; s < 2 * L::D; ++s) {
                 if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;
@@ -561,22 +596,24 @@ struct SolverImpl : Impl {
                 upload(prevf[0][s], pf);
                 upload(prevg[0][s], pg);
             }
-            StepArgs<ST, CT> a = args(0, 0, 0);  // pull 0 -> 1; u_old = u[0] -> u[1]; prev 0 -> 1
+            StepArgs<ST, CT> a = args(0, 0, 0, phib[1], phib[0]);
             launch_phase_pull(a);
             launch_fused_pull<MODE_FULL>(a);
             HIP_CHECK(hipStreamSynchronize(stream));
             cur = 1;
             ucur = 1;
             pcur = 1;
+            phc = 1;
             direct_old = false;
+            old_ref_zero = false;
             direct_f_interior.clear();
             direct_g_interior.clear();
             initialized = true;
             return;
         }
-        // Reference restart (main.cu:67-71): buffer 0 holds dev_f / dev_g with
-        // the interior replaced by f_post / g_post (read directly), buffer 1
-        // dev_f / dev_g; macroscopic() with u_old = 0, then the collision.
+        // Reference restart (main.cu:67-71): pop 0 holds dev_f / dev_g with the
+        // interior replaced by f_post / g_post (read directly, absolute),
+        // pop 1 dev_f / dev_g; macroscopic() with u_old = 0, then the collision.
         std::vector<double> a0 = f, b0 = gg;
         for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
             for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
@@ -586,16 +623,18 @@ struct SolverImpl : Impl {
                         a0[t] = fpost[t];
                         b0[t] = gpost[t];
                     }
-        upload_pop(fbuf[0], a0);
-        upload_pop(gbuf[0], b0);
-        upload_pop(fbuf[1], f);
-        upload_pop(gbuf[1], gg);
+        upload_pop(fbuf[0], a0, nullptr);
+        upload_pop(gbuf[0], b0, nullptr);
+        upload_pop(fbuf[1], f, nullptr);
+        upload_pop(gbuf[1], gg, nullptr);
         std::vector<double> zero((size_t)g.nodes * L::D, 0.0);
         upload(u[0], zero);
-        StepArgs<ST, CT> a = args(0, 0, 1);  // direct pull of 0 -> 1; u_old = u[0] -> u[1]; prev_out = prev[0]
+        StepArgs<ST, CT> a = args(0, 0, 1, phib[1], nullptr);
         k_phase<L, ST, CT, SP, true, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
         launch_check("k_phase");
-        ghost_refresh();
+        // ghost ring from the resumed phi itself
+        HIP_CHECK(hipMemcpyAsync(phib[0], phib[1], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice, stream));
+        ghost_refresh(phib[0], phib[1]);
         k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
         launch_check("k_fused");
         k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(a, true, scratch, p);
@@ -604,7 +643,9 @@ struct SolverImpl : Impl {
         cur = 1;
         ucur = 1;
         pcur = 0;
+        phc = 1;
         direct_old = true;
+        old_ref_zero = true;
         direct_f_interior = f;
         direct_g_interior = gg;
         initialized = true;
@@ -613,14 +654,9 @@ struct SolverImpl : Impl {
     void uold_soa(std::vector<double>& out) override { download(u[1 - ucur], (size_t)g.nodes * L::D, out); }
 
     double reduce(int field, unsigned* nonfinite) override {
-        const CT* src = phi;
-        if (field == CLIP_FIELD_RHO) {
-            // rho is a function of phi; sum phi and map
-            src = phi;
-        }
         HIP_CHECK(hipMemsetAsync(red_bad, 0, sizeof(unsigned), stream));
         const int blocks = 1024;
-        k_reduce_interior<CT><<<blocks, 256, 0, stream>>>(src, g, L::D, red_part, red_bad, 0.0);
+        k_reduce_interior<CT><<<blocks, 256, 0, stream>>>(phib[phc], g, red_part, red_bad);
         launch_check("k_reduce_interior");
         k_reduce_final<double><<<1, 256, 0, stream>>>(red_part, blocks, red_out);
         launch_check("k_reduce_final");
@@ -631,6 +667,7 @@ struct SolverImpl : Impl {
         HIP_CHECK(hipStreamSynchronize(stream));
         if (nonfinite) *nonfinite = bad;
         if (field == CLIP_FIELD_RHO) {
+            // rho is affine in phi
             const double ni = (double)(g.dmax[0] - g.dmin[0] + 1) * (g.dmax[1] - g.dmin[1] + 1) *
                               (g.dmax[2] - g.dmin[2] + 1);
             return cfg.rho_l * ni + s * (cfg.rho_h - cfg.rho_l);
