@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+POP_STORAGE = {"f64": "f64", "f32": "f32", "mixed_f16": "f16", "mixed_bf16": "bf16"}
+
+
 def pop_bytes(precision):
     return {"f64": 8, "f32": 4, "mixed_f16": 2, "mixed_bf16": 2}[precision]
 
@@ -90,6 +93,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run an elementwise FP32 copy (read + write) of this many MiB five times; "
+                         "its rocprofv3 FETCH_SIZE / WRITE_SIZE calibrate the traffic counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -101,6 +107,16 @@ def main():
     from clip_amd import Config, Solver
 
     torch.cuda.set_device(0)
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}),
+              flush=True)
+        return
     text = workload_text(args.workload)
     cfg = Config.from_string(text)
     sol = Solver(cfg, args.precision).init_fresh()
@@ -145,10 +161,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision.replace("mixed_", "") if args.precision.startswith("mixed") else args.precision,
+        "dtype": "f64" if args.precision == "f64" else "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "interior_nodes": interior, "lattice": f"D{D}Q{Q}",
-                   "precision": args.precision, "parallelism": "single-gpu"},
+                   "precision": args.precision, "population_storage": POP_STORAGE[args.precision],
+                   "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic("k_fused", args.workload, args.precision),

@@ -1030,6 +1030,12 @@ int orc_filter(int dim, double* v, double s) {
 
 double orc_sdf(const clip_config* c, double x, double y, double z) { return sdf(*c, c->dim, 0, x, y, z); }
 
+// calculateVF of one node with the handle's lattice and densities.
+int orc_viscous_force(void* h, const double* gneq, double tau, const double* dc, double* fv) {
+    viscous_force(*(Oracle*)h, gneq, tau, dc, fv);
+    return 0;
+}
+
 void orc_destroy(void* h) { delete (Oracle*)h; }
 
 }  // extern "C"

@@ -55,6 +55,7 @@ def load():
     lib.orc_gamma.argtypes = [ctypes.c_int, ctypes.c_int, D]
     lib.orc_destroy.restype = None
     lib.orc_destroy.argtypes = [P]
+    lib.orc_viscous_force.argtypes = [P, D, ctypes.c_double, D, D]
     _lib = lib
     return lib
 

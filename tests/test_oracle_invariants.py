@@ -450,3 +450,60 @@ def test_oracle_and_product_tables_agree(dim):
     assert np.abs(L["Minv"] - np.array([[float(x) for x in r] for r in Mi])).max() < 1e-16
     E = gl.D2Q9_E if dim == 2 else [e[:3] for e in gl.D3Q19_E]
     assert [tuple(r[:dim]) for r in L["e"]] == [tuple(e[:dim]) for e in E]
+
+
+# ------------------------------------------------------- viscous force
+@pytest.mark.parametrize("dim,mode", [(2, "xy"), (2, "xx-yy"), (3, "xy"), (3, "yz"), (3, "xx-yy")])
+def test_viscous_force_on_pure_stress_modes(dim, mode):
+    """calculateVF after the WMRT filter (NsAllen.cu:102-132, 416-431): for
+    gneq_q = A w_q P(e_q) with P a traceless second-order monomial, the
+    filter scales gneq by s and sum_q e e gneq' follows from the lattice's
+    fourth-order isotropy (sum w e_a^2 e_b^2 = 1/9, sum w e_a^4 = 1/3), so
+    fv = -tau drho Pi' grad(phi) is known in closed form."""
+    text = cases.drop2d(n=8, radius=2) if dim == 2 else cases.bubble3d(n=6, radius=2)
+    cfg = Config.from_string(text)
+    o = Oracle(cfg)
+    L = lat(dim)
+    e, w = L["e"], L["w"]
+    A, s, tau = 0.37, 0.61, 0.23
+    P = {"xy": e[:, 0] * e[:, 1], "yz": e[:, 1] * e[:, 2], "xx-yy": e[:, 0] ** 2 - e[:, 1] ** 2}[mode]
+    gneq = A * w * P
+    gf = oracle_lib.wmrt_filter(dim, gneq, s)
+    assert np.allclose(gf, s * gneq, rtol=0, atol=1e-15)
+    dc = np.array([0.3, -0.7, 0.5])
+    fv = np.zeros(3)
+    o.lib.orc_viscous_force(o.h, oracle_lib._dptr(gf), tau, oracle_lib._dptr(dc), oracle_lib._dptr(fv))
+    Pi = np.zeros((3, 3))
+    if mode == "xy":
+        Pi[0, 1] = Pi[1, 0] = s * A / 9
+    elif mode == "yz":
+        Pi[1, 2] = Pi[2, 1] = s * A / 9
+    else:
+        Pi[0, 0], Pi[1, 1] = 2 * s * A / 9, -2 * s * A / 9
+    dcd = dc.copy()
+    if dim == 2:
+        dcd[2] = 0.0
+    expect = -tau * (cfg.rho_h - cfg.rho_l) * (Pi @ dcd)
+    assert np.allclose(fv[:dim], expect[:dim], rtol=1e-13, atol=1e-18), (fv, expect)
+
+
+# ------------------------------------------------------------ Laplace law
+def test_static_drop_obeys_laplace_law():
+    """The coupled step relaxes a 2D drop to the Young-Laplace jump
+    P_in - P_out = sigma / R, with CLIP's pressure P = p rho / 3
+    (kernelMacroscopicf, NsAllen.cu:369-511; the surface force mu grad phi,
+    the pressure-gradient correction and the WMRT collision all enter)."""
+    R = 10.0
+    cfg = Config.from_string(cases.drop2d(n=48, radius=R, rho_ratio=10, mu_ratio=10))
+    o = Oracle(cfg).init()
+    o.step(3000)
+    phi = o.interior("phi")[..., 0, 0]
+    p = o.interior("p")[..., 0, 0]
+    rho = cfg.rho_l + phi * (cfg.rho_h - cfg.rho_l)
+    P = p * rho / 3.0
+    inside = P[phi > 0.99].mean()
+    outside = P[phi < 0.01].mean()
+    jump = inside - outside
+    assert abs(jump - cfg.sigma / R) < 0.08 * cfg.sigma / R, (jump, cfg.sigma / R)
+    # the drop stays put and round: spurious currents are small
+    assert np.abs(o.interior("u")).max() < 1e-2 * np.sqrt(cfg.sigma / R)
