@@ -83,6 +83,54 @@ def cpu_baseline(text, steps_cap, seconds_budget):
                       f"OpenMP over {cores} threads, {dt:.1f} s"}
 
 
+def adjoint_measure(precision, n=256, radius=50, steps=500, every=50):
+    """Reverse-mode cost on the 2D drop (D2Q9, n^2, R = radius, density ratio
+    1000): loss = 1/2 |phi_T - phi_target|^2 with the target from a run with
+    sigma x 1.2 and mobility x 0.8; phi0 = the case's initial phase field plus
+    1e-3 N(0,1) (seed 0) smoothed by one Laplacian pass and confined to the
+    interface band by 4 phi (1 - phi).  Times the plain forward (steps x
+    NSAllen::solve), the forward with states kept every `every` steps, and the
+    reverse sweep (which recomputes each segment once), with the interface
+    normal's derivative bounded on plateaus (normal floor 0.1)."""
+    import numpy as np
+    import torch
+    from clip_amd import Config, Solver
+    from tests import cases
+
+    cfg = Config.from_string(cases.drop2d(n=n, radius=radius))
+    base = Solver(cfg, precision).init_fresh()
+    phi0 = base.interior("phi")[..., 0, 0]
+    rng = np.random.default_rng(0)
+    xi = rng.standard_normal(phi0.shape)
+    lap = sum(np.roll(xi, s_, a) for a in (0, 1) for s_ in (1, -1)) - 4 * xi
+    phi0 = phi0 + 1e-3 * (xi + 0.25 * lap) * 4 * phi0 * (1 - phi0)
+    base.step(10)
+    base.synchronize()
+    t0 = time.perf_counter()
+    base.step(steps)
+    base.synchronize()
+    fwd = time.perf_counter() - t0
+    base.close()
+    target = Solver(cfg.with_params(sigma=1.2 * cfg.sigma, mobility=0.8 * cfg.mobility),
+                    precision).adjoint_forward(phi0, steps, steps)
+    s = Solver(cfg, precision)
+    s.adjoint_forward(phi0, 20, 10)  # allocation and first-launch costs outside the timed region
+    s.adjoint_backward(np.zeros_like(phi0), 0.1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phiT = s.adjoint_forward(phi0, steps, every)
+    t1 = time.perf_counter()
+    g0, gs, gm = s.adjoint_backward(phiT - target, 0.1)
+    t2 = time.perf_counter()
+    s.close()
+    return {"workload": f"drop2d{n}_adjoint", "precision": precision, "lattice": "D2Q9", "steps": steps,
+            "checkpoint_every": every, "normal_floor": 0.1, "forward_ms": round(fwd * 1e3, 3),
+            "forward_with_checkpoints_ms": round((t1 - t0) * 1e3, 3), "backward_ms": round((t2 - t1) * 1e3, 3),
+            "adjoint_over_forward": round((t2 - t1) / fwd, 3),
+            "loss": float(0.5 * np.sum((phiT - target) ** 2)), "grad_sigma": gs, "grad_mobility": gm,
+            "grad_phi0_norm": float(np.linalg.norm(g0))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +141,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run an elementwise FP32 copy (read + write) of this many MiB five times; "
                          "its rocprofv3 FETCH_SIZE / WRITE_SIZE calibrate the traffic counters")
@@ -175,6 +224,8 @@ def main():
         "kernel_ms_per_step": {k: round(v / tsteps, 4) for k, v in tm.items()},
         "phi_sum": health,
     }
+    if not args.no_adjoint:
+        line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64")]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(text, args.cpu_steps, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -79,6 +79,21 @@ class Config:
             raise AttributeError(name)
         return getattr(self.raw, name)
 
+    def with_params(self, sigma=None, mobility=None):
+        """Copy with the surface tension and/or the mobility replaced
+        (derived beta, kappa follow: InputData.cu:76-114)."""
+        c = self.copy()
+        if sigma is not None:
+            c.raw.has_sigma = 1
+            c.raw.sigma_override = float(sigma)
+        if mobility is not None:
+            c.raw.mobility = float(mobility)
+        return c.derive()
+
+    def interior_shape(self):
+        ex = self.extents()
+        return tuple(e - 2 for e in ex[: self.raw.dim])
+
     def set(self, **kw):
         for k, v in kw.items():
             cur = getattr(self.raw, k)
@@ -190,6 +205,34 @@ class Solver:
 
     def enable_timing(self, on=True):
         _abi.check(self._lib.clip_enable_timing(self._h, 1 if on else 0))
+
+    # ---- reverse mode (fully periodic domains, "f64" / "f32" handles) ----
+    def adjoint_forward(self, phi0, nsteps, checkpoint_every=50):
+        """Runs nsteps steps from the interior phase field phi0 (array of the
+        interior shape, index [i, j(, k)]) and returns phi after the last
+        step (same shape); keeps states for adjoint_backward."""
+        shape = self.config.interior_shape()
+        a = np.ascontiguousarray(np.asarray(phi0, dtype=np.float64).reshape(shape).transpose()).ravel()
+        out = np.empty_like(a)
+        _abi.check(self._lib.clip_adjoint_forward(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                  int(nsteps), int(checkpoint_every),
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out.reshape(shape[::-1]).transpose().copy()
+
+    def adjoint_backward(self, grad_phi, normal_floor=0.0):
+        """dL/dphi0, dL/dsigma, dL/dmobility for dL/dphi(last step) = grad_phi.
+        normal_floor = 0: exact derivative of the steps; > 0 (e.g. 0.1):
+        bounded derivative of the interface normal on flat plateaus
+        (clip_adjoint_set_normal_floor)."""
+        _abi.check(self._lib.clip_adjoint_set_normal_floor(self._h, float(normal_floor)))
+        shape = self.config.interior_shape()
+        g = np.ascontiguousarray(np.asarray(grad_phi, dtype=np.float64).reshape(shape).transpose()).ravel()
+        out = np.empty_like(g)
+        gs, gm = ctypes.c_double(), ctypes.c_double()
+        _abi.check(self._lib.clip_adjoint_backward(self._h, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   ctypes.byref(gs), ctypes.byref(gm)))
+        return out.reshape(shape[::-1]).transpose().copy(), gs.value, gm.value
 
     def last_timing(self):
         vals = [ctypes.c_double() for _ in range(4)]

@@ -55,6 +55,12 @@ struct Impl {
     virtual void set_stream(hipStream_t s) = 0;
     virtual void timing(double* total, double* ghost, double* phase, double* fused) const = 0;
     virtual void enable_timing(bool on) = 0;
+    // Reverse mode from an interior phi0 (x fastest) over nsteps steps.
+    virtual void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every,
+                                 std::vector<double>& phi_out) = 0;
+    virtual void adjoint_options(double normal_floor) = 0;
+    virtual void adjoint_backward(const std::vector<double>& grad_out, std::vector<double>& grad_phi0,
+                                  double* grad_sigma, double* grad_mobility) = 0;
     int D = 2, Q = 9;
     unsigned long long bytes = 0;
     long long step_count = 0;

@@ -29,7 +29,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
-#include "lattice.hpp"
+#include "node.hpp"
 
 namespace clipamd {
 
@@ -79,12 +79,6 @@ __host__ __device__ inline long long plane_size(const DevGrid& g, int side) {
     return (long long)g.ext[t1] * g.ext[t2];
 }
 
-template <class CT>
-struct Phys {
-    CT rho_h, rho_l, tau_h, tau_l, beta, kappa, mobility, W;
-    CT grav[3];
-    int case_type;
-};
 
 // ------------------------------------------------------------ storage
 // Storage policies convert between the stored population and the arithmetic
@@ -283,6 +277,25 @@ __device__ inline void store_prev(const DevGrid& g, const DevBC& bc, CT* const* 
 }
 
 // ------------------------------------------------------------ node math
+// Quotient x / d from r = 1/d (r correctly rounded): q0 = x r, residual
+// x - q0 d exact by FMA, q = q0 + residual r (Markstein, IBM J. Res. Dev.
+// 34(1), 1990).  Equals the IEEE quotient for the operands of this path
+// (checked against plain division by the CPU tests), so the FP64 results stay
+// identical to the oracle's while each reuse of a divisor costs 3 operations.
+template <class T>
+__device__ __forceinline__ T div_rn(T x, T d, T r) {
+    const T q0 = x * r;
+    const T rem = __builtin_fma(-q0, d, x);
+    return __builtin_fma(rem, r, q0);
+}
+// Division by a compile-time constant: exact quotient in FP64, multiplication
+// by the reciprocal in FP32.
+template <class CT>
+__device__ __forceinline__ CT cdiv(CT x, double d) {
+    if constexpr (sizeof(CT) == 8) return div_rn(x, d, 1.0 / d);
+    else return x * CT(1.0 / d);
+}
+
 // Velocity part of the equilibrium, Equilibrium_new (NsAllen.cu:77-94).
 template <class L, class CT>
 __device__ __forceinline__ CT gamma_u(int q, const CT u[3]) {
@@ -296,37 +309,36 @@ __device__ __forceinline__ CT gamma_u(int q, const CT u[3]) {
 }
 
 // gneq <- M^-1 S M gneq, stress moments scaled by s (NsAllen.cu:416-431),
-// with the reference's transform tables (integer rows, M^-1 rows over a
-// common denominator, true division in FP64).
+// evaluated as gneq + (s - 1) M^-1[:, stress] (M gneq)[stress] with integer
+// rows of M and M^-1 numerators over a common denominator; the oracle uses
+// the same form and operation order.
 template <class L, class CT>
 __device__ __forceinline__ void wmrt_filter(CT v[L::Q], CT s) {
-    CT m[L::Q];
+    CT m[L::NSTRESS];
 #pragma unroll
-    for (int r = 0; r < L::Q; ++r) {
+    for (int k = 0; k < L::NSTRESS; ++k) {
         CT acc = CT(0);
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) {
-            const int c = L::M(r, q);
+            const int c = L::M(L::stress(k), q);
             if (c == 1) acc += v[q];
             else if (c == -1) acc -= v[q];
             else if (c != 0) acc += CT(c) * v[q];
         }
-        m[r] = acc;
+        m[k] = acc;
     }
-#pragma unroll
-    for (int k = 0; k < L::NSTRESS; ++k) m[L::stress(k)] *= s;
+    const CT sm1 = s - CT(1);
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
         CT acc = CT(0);
 #pragma unroll
-        for (int r = 0; r < L::Q; ++r) {
-            const int c = L::minv_num(q, r);
-            if (c == 1) acc += m[r];
-            else if (c == -1) acc -= m[r];
-            else if (c != 0) acc += CT(c) * m[r];
+        for (int k = 0; k < L::NSTRESS; ++k) {
+            const int c = L::minv_num(q, L::stress(k));
+            if (c == 1) acc += m[k];
+            else if (c == -1) acc -= m[k];
+            else if (c != 0) acc += CT(c) * m[k];
         }
-        if constexpr (sizeof(CT) == 8) v[q] = acc / CT(L::minv_den(q));
-        else v[q] = acc * (CT(1) / CT(L::minv_den(q)));
+        v[q] = v[q] + sm1 * cdiv<CT>(acc, double(L::minv_den(q)));
     }
 }
 
@@ -349,88 +361,6 @@ __device__ __forceinline__ void viscous_force(const CT gneq[L::Q], CT tau, CT dr
     fv[0] = -tau * (sxx * dc[0] + sxy * dc[1] + szx * dc[2]) * drho;
     fv[1] = -tau * (sxy * dc[0] + syy * dc[1] + szy * dc[2]) * drho;
     fv[2] = (L::D == 3) ? -tau * (szz * dc[2] + szx * dc[0] + szy * dc[1]) * drho : CT(0);
-}
-
-// ---- moment-space form of the filter (FP32 / mixed kernels) ----------------
-// filter(v) = v + (s-1) sum_k M^-1[:,k] m_k(v), k over the stress rows.  Its
-// second moments are Pi(v) + (s-1) sum_k m_k Pi(M^-1[:,k]); the forcing term
-// hlp_q ~ w_q e_q.F has zero stress moments (even rows, odd vector), so the
-// reference's two-pass collision reduces exactly to
-//   f*_q = p w_q + Gamma_q(u) + hlp_q/2 - (s-1) sum_k M^-1[q][k] m_k(gneq).
-template <class L>
-__host__ __device__ constexpr double minv_at(int q, int k) {
-    return double(L::minv_num(q, L::stress(k))) / double(L::minv_den(q));
-}
-template <class L>
-__host__ __device__ constexpr double pi_dual(int k, int a, int b) {
-    double acc = 0.0;
-    for (int q = 0; q < L::Q; ++q) acc += double(L::e(q, a) * L::e(q, b)) * minv_at<L>(q, k);
-    return acc;
-}
-
-// Stress moments m_k and second moments Pi_ab of v.
-template <class L, class CT>
-__device__ __forceinline__ void stress_moments(const CT v[L::Q], CT mk[L::NSTRESS], CT pi[3][3]) {
-#pragma unroll
-    for (int k = 0; k < L::NSTRESS; ++k) {
-        CT acc = CT(0);
-#pragma unroll
-        for (int q = 0; q < L::Q; ++q) {
-            const int c = L::M(L::stress(k), q);
-            if (c == 1) acc += v[q];
-            else if (c == -1) acc -= v[q];
-            else if (c != 0) acc += CT(c) * v[q];
-        }
-        mk[k] = acc;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) pi[a][b] = CT(0);
-#pragma unroll
-    for (int a = 0; a < L::D; ++a)
-#pragma unroll
-        for (int b = a; b < L::D; ++b) {
-            CT acc = CT(0);
-#pragma unroll
-            for (int q = 0; q < L::Q; ++q) {
-                const int c = L::e(q, a) * L::e(q, b);
-                if (c == 1) acc += v[q];
-                else if (c == -1) acc -= v[q];
-            }
-            pi[a][b] = acc;
-            pi[b][a] = acc;
-        }
-}
-
-// Viscous force from the filtered second moments.
-template <class L, class CT>
-__device__ __forceinline__ void viscous_force_moments(const CT mk[L::NSTRESS], const CT pi[3][3], CT s, CT tau,
-                                                      CT drho, const CT dc[3], CT fv[3]) {
-    CT pf[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            if (a >= L::D || b >= L::D) {
-                pf[a][b] = CT(0);
-                continue;
-            }
-            CT acc = CT(0);
-#pragma unroll
-            for (int k = 0; k < L::NSTRESS; ++k) {
-                const double cf = pi_dual<L>(k, a, b);
-                if (cf != 0.0) acc += CT(cf) * mk[k];
-            }
-            pf[a][b] = pi[a][b] + (s - CT(1)) * acc;
-        }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        CT acc = CT(0);
-#pragma unroll
-        for (int b = 0; b < L::D; ++b) acc += pf[a][b] * dc[b];
-        fv[a] = (a < L::D) ? -tau * acc * drho : CT(0);
-    }
 }
 
 template <class CT>
@@ -466,12 +396,12 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
         const long long sy = g.sy;
         const CT xp = phi[n + 1], xm = phi[n - 1], yp = phi[n + sy], ym = phi[n - sy];
         const CT pp = phi[n + 1 + sy], pm = phi[n + 1 - sy], mp = phi[n - 1 + sy], mm = phi[n - 1 - sy];
-        dc[0] = (xp - xm) / CT(3) + ((pm + pp) - (mm + mp)) / CT(12);
-        dc[1] = (yp - ym) / CT(3) + ((mp + pp) - (mm + pm)) / CT(12);
+        dc[0] = cdiv<CT>(xp - xm, 3.0) + cdiv<CT>((pm + pp) - (mm + mp), 12.0);
+        dc[1] = cdiv<CT>(yp - ym, 3.0) + cdiv<CT>((mp + pp) - (mm + pm), 12.0);
         dc[2] = CT(0);
         const CT corners = ((mm + mp) + pm) + pp;
         const CT edges = ((xm + ym) + yp) + xp;
-        lap = (corners + CT(4) * edges - CT(20) * c0) / CT(6);
+        lap = cdiv<CT>(corners + CT(4) * edges - CT(20) * c0, 6.0);
     } else {
         const long long sy = g.sy, sz = g.sz;
         CT v[3][3][3];
@@ -511,7 +441,7 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
                     else if (m == 2) edges += v[a][b][d];
                     else if (m == 3) corners += v[a][b][d];
                 }
-        lap = (CT(20) * faces + CT(6) * edges + corners - CT(200) * c0) / CT(48);
+        lap = cdiv<CT>(CT(20) * faces + CT(6) * edges + corners - CT(200) * c0, 48.0);
     }
 }
 
@@ -650,7 +580,7 @@ enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 // Fused macroscopic + collision (kernelMacroscopicf NsAllen.cu:369-511,
 // normal_FD 241-275, kernelCollisionMRTg 545-584, kernelCollisionMRTf 589-763).
 // FP64: the reference's transforms and operation order.  FP32 / mixed: the
-// exact moment-space form of the same collision (see stress_moments).
+// exact moment-space form of the same step (node.hpp).
 template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL>
 __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     constexpr int Q = L::Q;
@@ -666,12 +596,6 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     // phi stencil -> grad phi, chemical potential
     CT c0, dc[3], lap;
     phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap);
-    const CT mu = CT(4) * ph.beta * c0 * (c0 - CT(1)) * (c0 - CT(0.5)) - ph.kappa * lap;
-    const CT rho = ph.rho_l + c0 * (ph.rho_h - ph.rho_l);
-    const CT drho = ph.rho_h - ph.rho_l;
-    const CT drho3 = drho / CT(3);
-    const CT tau = tau_of(ph, c0);
-    const CT s = CT(1) / (tau + CT(0.5));
 
     PullOffsets po;
     if (!DIRECT && !SHELL) po = pull_offsets<L>(g, c);
@@ -693,121 +617,126 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
             for (int q = 0; q < Q; ++q) store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fp[q]);
         }
     }
+    // g_post
+    auto pull_g = [&](int q) -> CT {
+        if (DIRECT) return SP::ld(a.gin, (long long)q * g.nodes + n, q, gref<SP>(a.phi_ref, n));
+        if (!SHELL) {
+            const long long src = pull_src<L>(n, po, q);
+            return SP::ld(a.gin, (long long)q * g.nodes + src, q, gref<SP>(a.phi_ref, src));
+        }
+        Puller<L, ST, CT, SP, true> P{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
+        return P.lvl3(c, q);
+    };
 
-    CT fg[3];
-    body_force(ph, rho, fg);
-
-    CT p, u[3];
-    if (MODE == MODE_FULL) {
-        // macroscopic p, u (kernelMacroscopicf): gneq from u_old
-        if constexpr (MOMENT_FORM) {
+    if constexpr (MOMENT_FORM) {
+        CT p;
+        if (MODE == MODE_FULL) {
             double pd = 0.0;  // pressure recovered in FP64
 #pragma unroll
             for (int q = 0; q < Q; ++q) pd += (double)fp[q];
             p = (CT)pd;
         } else {
+            p = a.p[n];
+        }
+        CT mom[3], Pf[3][3];
+        pop_moments<L>(fp, mom, Pf);
+        const NodeScalars<CT> z = node_scalars<L>(ph, c0, lap);
+        CT u[3];
+        if (MODE == MODE_FULL) {
+            CT uo[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) uo[d] = (d < L::D) ? a.uin[(long long)d * g.nodes + n] : CT(0);
+            velocity_of<L>(z, p, mom, Pf, dc, uo, u);
+#pragma unroll
+            for (int d = 0; d < L::D; ++d) a.uout[(long long)d * g.nodes + n] = u[d];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? a.ucol[(long long)d * g.nodes + n] : CT(0);
+        }
+        CT gp[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            gp[q] = pull_g(q);
+            if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp[q]);
+        }
+        collide_node<L, CT>(
+            z, p, Pf, c0, dc, u, gp,
+            [&](int q, CT v) { SP::st(a.gout, (long long)q * g.nodes + n, q, v, c0); },
+            [&](int q, CT v) { SP::st(a.fout, (long long)q * g.nodes + n, q, v, CT(0)); });
+        return;
+    } else {
+        const CT mu = CT(4) * ph.beta * c0 * (c0 - CT(1)) * (c0 - CT(0.5)) - ph.kappa * lap;
+        const CT rho = ph.rho_l + c0 * (ph.rho_h - ph.rho_l);
+        const CT drho = ph.rho_h - ph.rho_l;
+        const CT drho3 = cdiv<CT>(drho, 3.0);
+        const CT rr = CT(1) / rho;
+        const CT tau = tau_of(ph, c0);
+        const CT s = CT(1) / (tau + CT(0.5));
+        CT fg[3];
+        body_force(ph, rho, fg);
+
+        CT p, u[3];
+        if (MODE == MODE_FULL) {
+            // macroscopic p, u (kernelMacroscopicf): gneq from u_old
             p = CT(0);
 #pragma unroll
             for (int q = 0; q < Q; ++q) p += fp[q];
-        }
-        CT uo[3];
+            CT uo[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) uo[d] = (d < L::D) ? a.uin[(long long)d * g.nodes + n] : CT(0);
-        CT gneq[Q];
+            for (int d = 0; d < 3; ++d) uo[d] = (d < L::D) ? a.uin[(long long)d * g.nodes + n] : CT(0);
+            CT gneq[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) gneq[q] = fp[q] - (p * CT(L::w(q)) + gamma_u<L, CT>(q, uo));
-        CT fv[3];
-        if constexpr (MOMENT_FORM) {
-            CT mk[L::NSTRESS], pi[3][3];
-            stress_moments<L, CT>(gneq, mk, pi);
-            viscous_force_moments<L, CT>(mk, pi, s, tau, drho, dc, fv);
-        } else {
+            for (int q = 0; q < Q; ++q) gneq[q] = fp[q] - (p * CT(L::w(q)) + gamma_u<L, CT>(q, uo));
+            CT fv[3];
             wmrt_filter<L, CT>(gneq, s);
             viscous_force<L, CT>(gneq, tau, drho, dc, fv);
-        }
-        CT F[3], mom[3] = {0, 0, 0};
+            CT F[3], mom[3] = {0, 0, 0};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
+            for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            if (L::ex(q)) mom[0] += CT(L::ex(q)) * fp[q];
-            if (L::ey(q)) mom[1] += CT(L::ey(q)) * fp[q];
-            if (L::ez(q)) mom[2] += CT(L::ez(q)) * fp[q];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? mom[d] + CT(0.5) * F[d] / rho : CT(0);
-#pragma unroll
-        for (int d = 0; d < L::D; ++d) a.uout[(long long)d * g.nodes + n] = u[d];
-    } else {
-        p = a.p[n];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? a.ucol[(long long)d * g.nodes + n] : CT(0);
-    }
-
-    // normal (normal_FD)
-    CT nr[3];
-    {
-        CT sq = dc[0] * dc[0] + dc[1] * dc[1];
-        if (L::D == 3) sq += dc[2] * dc[2];
-        const CT t = sqrt(sq) + CT(1e-32);
-        nr[0] = dc[0] / t;
-        nr[1] = dc[1] / t;
-        nr[2] = (L::D == 3) ? dc[2] / t : CT(0);
-    }
-
-    // g collision (kernelCollisionMRTg), one population at a time
-    {
-        const CT wc = CT(1) / (CT(0.5) + CT(3) * ph.mobility);
-        const CT aterm = (CT(3) * ph.mobility * (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5))))) / ph.W;
-        Puller<L, ST, CT, SP, true> P{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
-        const CT rn = DIRECT ? gref<SP>(a.phi_ref, n) : CT(0);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            CT gp;
-            if (DIRECT) gp = SP::ld(a.gin, (long long)q * g.nodes + n, q, rn);
-            else if (!SHELL) {
-                const long long src = pull_src<L>(n, po, q);
-                gp = SP::ld(a.gin, (long long)q * g.nodes + src, q, gref<SP>(a.phi_ref, src));
-            } else
-                gp = P.lvl3(c, q);
-            if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
-            CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
-            if (L::D == 3) en += CT(L::ez(q)) * nr[2];
-            const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);
-            SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc, c0);
-        }
-    }
-
-    // f collision (kernelCollisionMRTf)
-    if constexpr (MOMENT_FORM) {
-        CT ga[Q], gneq[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            ga[q] = gamma_u<L, CT>(q, u);
-            gneq[q] = fp[q] - (p * CT(L::w(q)) + ga[q]);
-        }
-        CT mk[L::NSTRESS], pi[3][3], fv[3];
-        stress_moments<L, CT>(gneq, mk, pi);
-        viscous_force_moments<L, CT>(mk, pi, s, tau, drho, dc, fv);
-        CT F[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) F[d] = mu * dc[d] + (-p * drho3 * dc[d]) + fv[d] + fg[d];
-        const CT h = CT(1.5) / rho;  // hlp_q / 2 = 1.5 w_q (e_q.F) / rho
-        const CT sm1 = s - CT(1);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            CT eF = CT(L::ex(q)) * F[0] + CT(L::ey(q)) * F[1];
-            if (L::D == 3) eF += CT(L::ez(q)) * F[2];
-            CT corr = CT(0);
-#pragma unroll
-            for (int k = 0; k < L::NSTRESS; ++k) {
-                const double cm = minv_at<L>(q, k);
-                if (cm != 0.0) corr += CT(cm) * mk[k];
+            for (int q = 0; q < Q; ++q) {
+                if (L::ex(q)) mom[0] += CT(L::ex(q)) * fp[q];
+                if (L::ey(q)) mom[1] += CT(L::ey(q)) * fp[q];
+                if (L::ez(q)) mom[2] += CT(L::ez(q)) * fp[q];
             }
-            const CT fs = p * CT(L::w(q)) + ga[q] + CT(L::w(q)) * h * eF - sm1 * corr;
-            SP::st(a.fout, (long long)q * g.nodes + n, q, fs, CT(0));
+#pragma unroll
+            for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? mom[d] + div_rn(CT(0.5) * F[d], rho, rr) : CT(0);
+#pragma unroll
+            for (int d = 0; d < L::D; ++d) a.uout[(long long)d * g.nodes + n] = u[d];
+        } else {
+            p = a.p[n];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) u[d] = (d < L::D) ? a.ucol[(long long)d * g.nodes + n] : CT(0);
         }
-    } else {
+
+        // normal (normal_FD)
+        CT nr[3];
+        {
+            CT sq = dc[0] * dc[0] + dc[1] * dc[1];
+            if (L::D == 3) sq += dc[2] * dc[2];
+            const CT t = sqrt(sq) + CT(1e-32);
+            const CT rt = CT(1) / t;
+            nr[0] = div_rn(dc[0], t, rt);
+            nr[1] = div_rn(dc[1], t, rt);
+            nr[2] = (L::D == 3) ? div_rn(dc[2], t, rt) : CT(0);
+        }
+
+        // g collision (kernelCollisionMRTg), one population at a time
+        {
+            const CT wc = CT(1) / (CT(0.5) + CT(3) * ph.mobility);
+            const CT aterm = (CT(3) * ph.mobility * (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5))))) / ph.W;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const CT gp = pull_g(q);
+                if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
+                CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
+                if (L::D == 3) en += CT(L::ez(q)) * nr[2];
+                const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);
+                SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc, c0);
+            }
+        }
+
+        // f collision (kernelCollisionMRTf)
         CT ga[Q], gneq[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -825,7 +754,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         for (int q = 0; q < Q; ++q) {
             CT eF = CT(L::ex(q)) * F[0] + CT(L::ey(q)) * F[1];
             if (L::D == 3) eF += CT(L::ez(q)) * F[2];
-            hlp[q] = CT(3) * CT(L::w(q)) * eF / rho;
+            hlp[q] = div_rn(CT(3) * CT(L::w(q)) * eF, rho, rr);
             gneq[q] = fp[q] - (p * CT(L::w(q)) + ga[q] - CT(0.5) * hlp[q]);
         }
         wmrt_filter<L, CT>(gneq, s);

@@ -125,6 +125,14 @@ size_t nodes_of(const clip_config& c) {
     return (size_t)e[0] * e[1] * e[2];
 }
 
+size_t interior_of(const clip_config& c) {
+    unsigned e[3];
+    extents(c, e);
+    size_t n = 1;
+    for (int a = 0; a < c.dim; ++a) n *= (size_t)(e[a] - 2);
+    return n;
+}
+
 }  // namespace
 
 extern "C" {
@@ -248,6 +256,33 @@ int clip_enable_timing(clip_solver* s, int on) {
     if (!s) return set_error(CLIP_ERR_ARG, "null handle");
     s->impl->enable_timing(on != 0);
     return CLIP_OK;
+}
+
+int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out) {
+    if (!s || !phi0 || !phi_out) return set_error(CLIP_ERR_ARG, "null argument");
+    if (nsteps < 1 || checkpoint_every < 1) return set_error(CLIP_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+    return guard([&] {
+        const size_t n = interior_of(s->cfg);
+        std::vector<double> in(phi0, phi0 + n), out;
+        s->impl->adjoint_forward(in, nsteps, checkpoint_every, out);
+        std::memcpy(phi_out, out.data(), n * sizeof(double));
+    });
+}
+
+int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* grad_phi0, double* grad_sigma,
+                          double* grad_mobility) {
+    if (!s || !grad_phi_out || !grad_phi0) return set_error(CLIP_ERR_ARG, "null argument");
+    return guard([&] {
+        const size_t n = interior_of(s->cfg);
+        std::vector<double> in(grad_phi_out, grad_phi_out + n), out;
+        s->impl->adjoint_backward(in, out, grad_sigma, grad_mobility);
+        std::memcpy(grad_phi0, out.data(), n * sizeof(double));
+    });
+}
+
+int clip_adjoint_set_normal_floor(clip_solver* s, double normal_floor) {
+    if (!s) return set_error(CLIP_ERR_ARG, "null handle");
+    return guard([&] { s->impl->adjoint_options(normal_floor); });
 }
 
 void clip_destroy(clip_solver* s) { delete s; }

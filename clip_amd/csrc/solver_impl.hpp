@@ -16,11 +16,13 @@
 
 #include "geometry.hpp"
 #include "impl.hpp"
+#include "adjoint.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace clipamd {
 
@@ -169,6 +171,7 @@ struct SolverImpl : Impl {
 
     ~SolverImpl() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        adj_free();
         for (void* ptr : allocations) (void)hipFree(ptr);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -291,6 +294,12 @@ struct SolverImpl : Impl {
         for (int k = 0; k < g.ext[2]; ++k)
             for (int j = 0; j < g.ext[1]; ++j)
                 for (int i = 0; i < g.ext[0]; ++i) h[g.idx(i, j, k)] = (CT)initial_phi(cfg, i, j, k);
+        init_with_phi(h);
+    }
+
+    // Initial state (KernelInitializeDistributions) from a phi given on every
+    // node, then the first collision.
+    void init_with_phi(const std::vector<CT>& h) {
         for (int b = 0; b < 2; ++b)
             HIP_CHECK(hipMemcpyAsync(phib[b], h.data(), h.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
         // Distributions, p, u on all nodes of both buffers; outflow planes.
@@ -649,6 +658,244 @@ struct SolverImpl : Impl {
         direct_f_interior = f;
         direct_g_interior = gg;
         initialized = true;
+    }
+
+    // ------------------------------------------------------------- adjoint
+    // Reverse mode over nsteps steps from a given interior phi0, with states
+    // kept every `every` steps and each segment recomputed in the sweep.
+    std::vector<void*> adj_allocs;
+    size_t adj_capacity = 0;  // CT elements in the state pool
+    CT* adj_pool = nullptr;
+    CT *lF = nullptr, *lG = nullptr, *lU = nullptr, *lPhi = nullptr, *lfp = nullptr, *lgp = nullptr,
+       *lst = nullptr, *lphin = nullptr;
+    double *adj_part = nullptr, *adj_acc = nullptr;
+    int adj_T = 0, adj_K = 0, adj_nsnap = 0;
+    double adj_nfloor = 0.0;
+    bool adj_ready = false;
+
+    size_t state_elems() const { return (size_t)g.nodes * (2 * QL + 1 + L::D); }
+    CT* snap_ptr(int k) { return adj_pool + (size_t)k * state_elems(); }
+    CT* seg_ptr(int j) { return adj_pool + (size_t)(adj_nsnap + j) * state_elems(); }
+    long long interior_count() const {
+        long long t = 1;
+        for (int a = 0; a < L::D; ++a) t *= g.dmax[a] - g.dmin[a] + 1;
+        return t;
+    }
+    template <class Fn>
+    void for_interior(Fn fn) const {  // fn(interior index, node index), x fastest
+        long long t = 0;
+        for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
+            for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
+                for (int i = g.dmin[0]; i <= g.dmax[0]; ++i) fn(t++, g.idx(i, j, k));
+    }
+    void adj_check() const {
+        if constexpr (!std::is_same<ST, CT>::value)
+            throw ConfigError("the adjoint runs on F64 or F32 handles");
+        for (int s = 0; s < 2 * L::D; ++s)
+            if (cfg.bc_type[s] != CLIP_BC_PERIODIC) throw ConfigError("the adjoint needs a fully periodic domain");
+    }
+    template <class T>
+    T* adj_alloc(size_t count) {
+        void* ptr = nullptr;
+        if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        adj_allocs.push_back(ptr);
+        return (T*)ptr;
+    }
+    void adj_free() {
+        for (void* p2 : adj_allocs) (void)hipFree(p2);
+        adj_allocs.clear();
+        adj_capacity = 0;
+        adj_pool = nullptr;
+    }
+    void save_state(CT* dst) {
+        static_assert(kPlain, "plain storage only");
+        const size_t nq = (size_t)g.nodes * QL;
+        HIP_CHECK(hipMemcpyAsync(dst, fbuf[cur], nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(dst + nq, gbuf[cur], nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(dst + 2 * nq, phib[phc], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice,
+                                 stream));
+        HIP_CHECK(hipMemcpyAsync(dst + 2 * nq + g.nodes, u[ucur], (size_t)g.nodes * L::D * sizeof(CT),
+                                 hipMemcpyDeviceToDevice, stream));
+    }
+    void load_state(const CT* src) {
+        const size_t nq = (size_t)g.nodes * QL;
+        HIP_CHECK(hipMemcpyAsync(fbuf[cur], src, nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(gbuf[cur], src + nq, nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(phib[phc], src + 2 * nq, (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice,
+                                 stream));
+        HIP_CHECK(hipMemcpyAsync(u[ucur], src + 2 * nq + g.nodes, (size_t)g.nodes * L::D * sizeof(CT),
+                                 hipMemcpyDeviceToDevice, stream));
+        direct_old = false;
+        old_ref_zero = false;
+    }
+
+    static constexpr bool kPlain = std::is_same<ST, CT>::value;
+
+    void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every, std::vector<double>& out) override {
+        adj_check();
+        if constexpr (kPlain) adjoint_forward_impl(phi0, nsteps, every, out);
+    }
+    void adjoint_forward_impl(const std::vector<double>& phi0, int nsteps, int every, std::vector<double>& out) {
+        if (nsteps < 1 || every < 1) throw ConfigError("adjoint: nsteps and checkpoint interval must be >= 1");
+        if ((long long)phi0.size() != interior_count()) throw ConfigError("adjoint: phi0 size != interior nodes");
+        // phi on every node: interior from phi0, ghosts by periodic wrap
+        std::vector<CT> h((size_t)g.nodes, CT(0));
+        for_interior([&](long long t, long long n) { h[n] = (CT)phi0[t]; });
+        for (int k = 0; k < g.ext[2]; ++k)
+            for (int j = 0; j < g.ext[1]; ++j)
+                for (int i = 0; i < g.ext[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    if (g.interior(c, L::D)) continue;
+                    int w[3] = {i, j, k};
+                    for (int a = 0; a < L::D; ++a) {
+                        const int span = g.dmax[a] - g.dmin[a] + 1;
+                        if (w[a] < g.dmin[a]) w[a] += span;
+                        if (w[a] > g.dmax[a]) w[a] -= span;
+                    }
+                    h[g.idx(i, j, k)] = h[g.idx(w[0], w[1], w[2])];
+                }
+        const int nsnap = (nsteps + every - 1) / every;
+        const size_t need = (size_t)(nsnap + every) * state_elems();
+        if (need > adj_capacity) {
+            adj_free();
+            adj_pool = adj_alloc<CT>(need);
+            adj_capacity = need;
+            const size_t nq = (size_t)g.nodes * QL;
+            lF = adj_alloc<CT>(nq);
+            lG = adj_alloc<CT>(nq);
+            lfp = adj_alloc<CT>(nq);
+            lgp = adj_alloc<CT>(nq);
+            lU = adj_alloc<CT>((size_t)g.nodes * L::D);
+            lPhi = adj_alloc<CT>((size_t)g.nodes);
+            lphin = adj_alloc<CT>((size_t)g.nodes);
+            lst = adj_alloc<CT>((size_t)g.nodes * (L::D + 2));
+            const long long blocks = (interior_count() + ADJ_TB - 1) / ADJ_TB;
+            adj_part = adj_alloc<double>((size_t)blocks * 4);
+            adj_acc = adj_alloc<double>(4);
+        }
+        adj_T = nsteps;
+        adj_K = every;
+        adj_nsnap = nsnap;
+        init_with_phi(h);
+        save_state(snap_ptr(0));
+        for (int n = 1; n <= nsteps; ++n) {
+            step(1);
+            if (n % every == 0 && n < nsteps) save_state(snap_ptr(n / every));
+        }
+        std::vector<double> phi;
+        download(phib[phc], (size_t)g.nodes, phi);
+        out.assign((size_t)interior_count(), 0.0);
+        for_interior([&](long long t, long long n) { out[t] = phi[n]; });
+        adj_ready = true;
+        step_count = nsteps;
+    }
+
+    AdjArgs<CT> adj_args(const CT* phin) {
+        AdjArgs<CT> A;
+        A.g = g;
+        A.ph = ph;
+        A.F = fbuf[cur];
+        A.G = gbuf[cur];
+        A.phin = phin;
+        A.uo = u[ucur];
+        A.lF = lF;
+        A.lG = lG;
+        A.lU = lU;
+        A.lPhi = lPhi;
+        A.lfp = lfp;
+        A.lgp = lgp;
+        A.lst = lst;
+        A.lphin = lphin;
+        A.part = adj_part;
+        A.sigma = (CT)cfg.sigma;
+        A.half_length = (CT)(cfg.reference_length / 2.0);
+        A.nfloor = (CT)adj_nfloor;
+        return A;
+    }
+
+    // Reverse of the step that maps the loaded state S^n to S^{n+1}.
+    void adjoint_step() {
+        CT* phi_old = phib[phc];
+        CT* phi_new = phib[1 - phc];
+        ghost_refresh(phi_old, phi_new);
+        launch_phase_pull(args(cur, ucur, pcur, phi_new, phi_old));
+        const AdjArgs<CT> A = adj_args(phi_new);
+        const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
+        k_adj_node<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
+        launch_check("k_adj_node");
+        k_adj_accumulate<CT><<<1, 64, 0, stream>>>(adj_part, (int)blocks, adj_acc);
+        launch_check("k_adj_accumulate");
+        k_adj_phi<L, CT, true><<<blocks, ADJ_TB, 0, stream>>>(A);
+        launch_check("k_adj_phi");
+        k_adj_pull<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
+        launch_check("k_adj_pull");
+    }
+
+    void adjoint_options(double normal_floor) override {
+        if (!(normal_floor >= 0.0)) throw ConfigError("adjoint: normal floor must be >= 0");
+        adj_nfloor = normal_floor;
+    }
+
+    void adjoint_backward(const std::vector<double>& gout, std::vector<double>& gphi0, double* gsigma,
+                          double* gmob) override {
+        adj_check();
+        if constexpr (kPlain) adjoint_backward_impl(gout, gphi0, gsigma, gmob);
+    }
+    void adjoint_backward_impl(const std::vector<double>& gout, std::vector<double>& gphi0, double* gsigma,
+                               double* gmob) {
+        if (!adj_ready) throw ConfigError("adjoint: call the adjoint forward first");
+        if ((long long)gout.size() != interior_count()) throw ConfigError("adjoint: gradient size != interior nodes");
+        const size_t nq = (size_t)g.nodes * QL;
+        HIP_CHECK(hipMemsetAsync(lF, 0, nq * sizeof(CT), stream));
+        HIP_CHECK(hipMemsetAsync(lG, 0, nq * sizeof(CT), stream));
+        HIP_CHECK(hipMemsetAsync(lU, 0, (size_t)g.nodes * L::D * sizeof(CT), stream));
+        HIP_CHECK(hipMemsetAsync(lst, 0, (size_t)g.nodes * (L::D + 2) * sizeof(CT), stream));
+        HIP_CHECK(hipMemsetAsync(lphin, 0, (size_t)g.nodes * sizeof(CT), stream));
+        HIP_CHECK(hipMemsetAsync(adj_acc, 0, 4 * sizeof(double), stream));
+        {
+            std::vector<double> lp((size_t)g.nodes, 0.0);
+            for_interior([&](long long t, long long n) { lp[n] = gout[t]; });
+            upload(lPhi, lp);
+        }
+        for (int k = adj_nsnap - 1; k >= 0; --k) {
+            const int s0 = k * adj_K;
+            const int s1 = std::min(adj_T, s0 + adj_K);
+            load_state(snap_ptr(k));
+            save_state(seg_ptr(0));
+            for (int j = 1; j < s1 - s0; ++j) {
+                step(1);
+                save_state(seg_ptr(j));
+            }
+            for (int n = s1 - 1; n >= s0; --n) {
+                load_state(seg_ptr(n - s0));
+                adjoint_step();
+            }
+        }
+        // initial state and first collision
+        load_state(snap_ptr(0));
+        const AdjArgs<CT> A = adj_args(phib[phc]);
+        const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
+        k_adj_init<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
+        launch_check("k_adj_init");
+        k_adj_accumulate<CT><<<1, 64, 0, stream>>>(adj_part, (int)blocks, adj_acc);
+        launch_check("k_adj_accumulate");
+        k_adj_phi<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A);
+        launch_check("k_adj_phi");
+        std::vector<double> lp;
+        download(lphin, (size_t)g.nodes, lp);
+        gphi0.assign((size_t)interior_count(), 0.0);
+        for_interior([&](long long t, long long n) { gphi0[t] = lp[n]; });
+        double acc[4];
+        HIP_CHECK(hipMemcpyAsync(acc, adj_acc, sizeof(acc), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        // beta = 8 sigma / W, kappa = 1.5 sigma W (InputData.cu:76-114)
+        if (gsigma) *gsigma = acc[0] * 8.0 / cfg.interface_width + acc[1] * 1.5 * cfg.interface_width + acc[3];
+        if (gmob) *gmob = acc[2];
+        // leave the handle at the final state of the forward run
+        adj_ready = false;
     }
 
     void uold_soa(std::vector<double>& out) override { download(u[1 - ucur], (size_t)g.nodes * L::D, out); }

@@ -200,6 +200,29 @@ int clip_get_info(const clip_solver* s, clip_info* out);
 int clip_last_step_timing(const clip_solver* s, double* total_ms, double* ghost_ms,
                           double* phase_ms, double* fused_ms);
 int clip_enable_timing(clip_solver* s, int on);
+/* ---- reverse mode (adjoint) ----------------------------------------------
+ * No counterpart in the reference: the differentiable-solver extension for
+ * gradient-based callers (parameter fitting, inverse problems).  Fully
+ * periodic domains, CLIP_PREC_F64 or CLIP_PREC_F32 handles.
+ * clip_adjoint_forward builds the initial state of the case
+ * (KernelInitializeDistributions, NsAllen.cu:137-202, with the first
+ * collision) from phi0 on the interior nodes (x fastest; ghosts by periodic
+ * wrap), runs nsteps steps keeping the state every checkpoint_every steps,
+ * and writes phi after the last step (interior, x fastest) to phi_out.
+ * clip_adjoint_backward takes dL/dphi_out and returns dL/dphi0 and, if the
+ * pointers are non-NULL, dL/dsigma (through beta, kappa and the initial p)
+ * and dL/dmobility; it recomputes each segment between kept states once. */
+int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out);
+int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* grad_phi0, double* grad_sigma,
+                          double* grad_mobility);
+/* Regularisation of the interface normal's derivative in the reverse sweep
+ * (no counterpart in the reference).  0 (default): the exact derivative of
+ * the step.  c > 0: where |grad phi| < c 4 phi (1 - phi) / W the derivative
+ * through n = grad phi / |grad phi| is scaled by |grad phi| / (c 4 phi (1 - phi) / W),
+ * which bounds it on flat plateaus (drop centres, far field) where the exact
+ * derivative ~ 1 / |grad phi| amplifies rounding without bound over long
+ * horizons; elsewhere the gradient is unchanged.  c = 0.1 is a good choice. */
+int clip_adjoint_set_normal_floor(clip_solver* s, double normal_floor);
 void clip_destroy(clip_solver* s);
 const char* clip_last_error(void);
 

@@ -267,22 +267,26 @@ inline double gamma_u(const Lattice& L, int q, const double* u) {
     return L.w[q] * (3.0 * eu + 4.5 * eu * eu - 1.5 * u2);
 }
 
-// gneq -> M gneq, stress moments scaled by s, -> M^-1 (NsAllen.cu:416-431).
+// The WMRT filter of NsAllen.cu:416-431 (m = M gneq, stress moments scaled
+// by s, back-transform with M^-1), evaluated as the identity plus its
+// stress-mode part: gneq + (s - 1) M^-1[:, stress] m[stress].  Same linear map
+// (M^-1 M = I); only the stress rows of M and columns of M^-1 are touched.
 void wmrt_filter(const Lattice& L, double* v, double s) {
-    double m[19];
-    for (int r = 0; r < L.Q; ++r) {
+    double m[5];
+    for (int k = 0; k < L.nstress; ++k) {
         double acc = 0;
-        for (int q = 0; q < L.Q; ++q) acc += L.M[r][q] * v[q];
-        m[r] = acc;
+        for (int q = 0; q < L.Q; ++q) acc += L.M[L.stress[k]][q] * v[q];
+        m[k] = acc;
     }
-    for (int k = 0; k < L.nstress; ++k) m[L.stress[k]] *= s;
+    const double sm1 = s - 1.0;
     for (int q = 0; q < L.Q; ++q) {
         double acc = 0;
-        for (int r = 0; r < L.Q; ++r) acc += L.Minv_num[q][r] * m[r];
-        v[q] = acc / L.Minv_den[q];
+        for (int k = 0; k < L.nstress; ++k) acc += L.Minv_num[q][L.stress[k]] * m[k];
+        v[q] = v[q] + sm1 * (acc / L.Minv_den[q]);
     }
 }
 
+// tau(phi) with the clamp of NsAllen.cu:418-426.
 inline double tau_of(const clip_config& c, double phi) {
     if (phi < 0.0) return c.tau_l;
     if (phi > 1.0) return c.tau_h;
@@ -899,6 +903,16 @@ void* orc_create(const clip_config* cfg) {
 int orc_init(void* h) {
     Oracle* o = (Oracle*)h;
     initial_phi(*o);
+    device_initializer(*o);
+    o->step = 0;
+    return 0;
+}
+
+// Same, with phi given on every node (CLIP AoS order, ghosts included)
+// instead of the case's initial condition.
+int orc_init_phi(void* h, const double* phi) {
+    Oracle* o = (Oracle*)h;
+    for (size_t i = 0; i < o->phi.size(); ++i) o->phi[i] = phi[i];
     device_initializer(*o);
     o->step = 0;
     return 0;

@@ -1,0 +1,136 @@
+// node_check.hip — host build of the per-node arithmetic (node.hpp) for the
+// CPU tests: forward node update and its reverse, FP64, on D2Q9 and D3Q19.
+// phys = [rho_h, rho_l, tau_h, tau_l, beta, kappa, mobility, W, g0, g1, g2, case_type].
+#include "node.hpp"
+
+using namespace clipamd;
+
+namespace {
+
+Phys<double> unpack(const double* a) {
+    Phys<double> ph;
+    ph.rho_h = a[0];
+    ph.rho_l = a[1];
+    ph.tau_h = a[2];
+    ph.tau_l = a[3];
+    ph.beta = a[4];
+    ph.kappa = a[5];
+    ph.mobility = a[6];
+    ph.W = a[7];
+    ph.grav[0] = a[8];
+    ph.grav[1] = a[9];
+    ph.grav[2] = a[10];
+    ph.case_type = (int)a[11];
+    return ph;
+}
+
+template <class L>
+void fwd(const double* phys, const double* fp, const double* gp, double c0, const double* dc, double lap,
+         const double* uo, double* fs, double* gs, double* u) {
+    const Phys<double> ph = unpack(phys);
+    double p = 0;
+    for (int q = 0; q < L::Q; ++q) p += fp[q];
+    double mom[3], Pf[3][3];
+    pop_moments<L>(fp, mom, Pf);
+    const NodeScalars<double> z = node_scalars<L>(ph, c0, lap);
+    double dcc[3] = {dc[0], dc[1], L::D == 3 ? dc[2] : 0.0};
+    double uoo[3] = {uo[0], uo[1], L::D == 3 ? uo[2] : 0.0};
+    double uu[3];
+    velocity_of<L>(z, p, mom, Pf, dcc, uoo, uu);
+    for (int d = 0; d < L::D; ++d) u[d] = uu[d];
+    collide_node<L, double>(
+        z, p, Pf, c0, dcc, uu, gp, [&](int q, double v) { gs[q] = v; }, [&](int q, double v) { fs[q] = v; });
+}
+
+template <class L>
+void rev(const double* phys, const double* fp, const double* gp, double c0, const double* dc, double lap,
+         const double* uo, const double* lfs, const double* lgs, const double* lu, double* lfp, double* lgp,
+         double* lin /* lc0, ldc[3], llap, luo[3], lbeta, lkappa, lmob */) {
+    const Phys<double> ph = unpack(phys);
+    double dcc[3] = {dc[0], dc[1], L::D == 3 ? dc[2] : 0.0};
+    double uoo[3] = {uo[0], uo[1], L::D == 3 ? uo[2] : 0.0};
+    double luu[3] = {lu[0], lu[1], L::D == 3 ? lu[2] : 0.0};
+    NodeAdj<L, double> o{};
+    node_reverse<L, double>(ph, fp, gp, c0, dcc, lap, uoo, lfs, lgs, luu, o);
+    for (int q = 0; q < L::Q; ++q) {
+        lfp[q] = o.lfp[q];
+        lgp[q] = o.lgp[q];
+    }
+    lin[0] = o.lc0;
+    for (int d = 0; d < 3; ++d) lin[1 + d] = o.ldc[d];
+    lin[4] = o.llap;
+    for (int d = 0; d < 3; ++d) lin[5 + d] = o.luo[d];
+    lin[8] = o.lbeta;
+    lin[9] = o.lkappa;
+    lin[10] = o.lmob;
+}
+
+// Initial state + first collision at one node (u = 0, initial p).
+template <class L>
+void init_fwd(const double* phys, double sigma, double half, double c0, const double* dc, double lap, double* fs,
+              double* gs) {
+    const Phys<double> ph = unpack(phys);
+    const NodeScalars<double> z = node_scalars<L>(ph, c0, lap);
+    const double k = ph.case_type == CLIP_CASE_DROP ? 1.0 : (ph.case_type == CLIP_CASE_BUBBLE ? -1.0 : 0.0);
+    const double p = k * c0 * sigma / half / (z.rho / 3.0);
+    double dcc[3] = {dc[0], dc[1], L::D == 3 ? dc[2] : 0.0};
+    double nr[3], rt, mag;
+    normal_of<L>(dcc, nr, rt, mag);
+    double fp[L::Q], gp[L::Q];
+    const double gi = 2.0 * c0 * (1.0 - c0) / ph.W;
+    for (int q = 0; q < L::Q; ++q) {
+        fp[q] = p * L::w(q);
+        gp[q] = L::w(q) * (c0 - gi * edot<L>(q, nr));
+    }
+    double mom[3], Pf[3][3];
+    pop_moments<L>(fp, mom, Pf);
+    const double u0[3] = {0, 0, 0};
+    collide_node<L, double>(
+        z, p, Pf, c0, dcc, u0, gp, [&](int q, double v) { gs[q] = v; }, [&](int q, double v) { fs[q] = v; });
+}
+
+template <class L>
+void init_rev(const double* phys, double sigma, double half, double c0, const double* dc, double lap,
+              const double* lfs, const double* lgs, double* lin /* lc0, ldc[3], llap, lbeta, lkappa, lmob, lsigma */) {
+    const Phys<double> ph = unpack(phys);
+    double dcc[3] = {dc[0], dc[1], L::D == 3 ? dc[2] : 0.0};
+    double lc0 = 0, ldc[3] = {0, 0, 0}, llap = 0, lb = 0, lk = 0, lm = 0, ls = 0;
+    init_reverse<L, double>(ph, sigma, half, c0, dcc, lap, lfs, lgs, lc0, ldc, llap, lb, lk, lm, ls);
+    lin[0] = lc0;
+    for (int d = 0; d < 3; ++d) lin[1 + d] = ldc[d];
+    lin[4] = llap;
+    lin[5] = lb;
+    lin[6] = lk;
+    lin[7] = lm;
+    lin[8] = ls;
+}
+
+}  // namespace
+
+extern "C" {
+int node_fwd(int dim, const double* phys, const double* fp, const double* gp, double c0, const double* dc, double lap,
+             const double* uo, double* fs, double* gs, double* u) {
+    if (dim == 2) fwd<D2Q9>(phys, fp, gp, c0, dc, lap, uo, fs, gs, u);
+    else fwd<D3Q19>(phys, fp, gp, c0, dc, lap, uo, fs, gs, u);
+    return 0;
+}
+int node_rev(int dim, const double* phys, const double* fp, const double* gp, double c0, const double* dc,
+             double lap, const double* uo, const double* lfs, const double* lgs, const double* lu, double* lfp,
+             double* lgp, double* lin) {
+    if (dim == 2) rev<D2Q9>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, lfp, lgp, lin);
+    else rev<D3Q19>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, lfp, lgp, lin);
+    return 0;
+}
+int init_fwd(int dim, const double* phys, double sigma, double half, double c0, const double* dc, double lap,
+             double* fs, double* gs) {
+    if (dim == 2) init_fwd<D2Q9>(phys, sigma, half, c0, dc, lap, fs, gs);
+    else init_fwd<D3Q19>(phys, sigma, half, c0, dc, lap, fs, gs);
+    return 0;
+}
+int init_rev(int dim, const double* phys, double sigma, double half, double c0, const double* dc, double lap,
+             const double* lfs, const double* lgs, double* lin) {
+    if (dim == 2) init_rev<D2Q9>(phys, sigma, half, c0, dc, lap, lfs, lgs, lin);
+    else init_rev<D3Q19>(phys, sigma, half, c0, dc, lap, lfs, lgs, lin);
+    return 0;
+}
+}

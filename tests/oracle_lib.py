@@ -56,6 +56,7 @@ def load():
     lib.orc_destroy.restype = None
     lib.orc_destroy.argtypes = [P]
     lib.orc_viscous_force.argtypes = [P, D, ctypes.c_double, D, D]
+    lib.orc_init_phi.argtypes = [P, D]
     _lib = lib
     return lib
 
@@ -81,6 +82,12 @@ class Oracle:
 
     def init(self):
         self.lib.orc_init(self.h)
+        return self
+
+    def init_from_phi(self, phi_grid):
+        """Initial state from phi on every node (grid shape, ghosts included)."""
+        a = np.ascontiguousarray(phi_grid, dtype=np.float64).ravel()
+        self.lib.orc_init_phi(self.h, _dptr(a))
         return self
 
     def step(self, n=1):

@@ -1,0 +1,177 @@
+"""Hand-written adjoint (clip_adjoint_forward / clip_adjoint_backward)
+checked against (1) torch autograd on the float64 CPU restatement
+(tests/torch_step.py, itself pinned to the oracle by test_torch_tape.py),
+(2) central finite differences of the FP64 GPU forward, (3) FP32 against
+FP64, and (4) through the torch.autograd.Function."""
+import numpy as np
+import pytest
+import torch
+
+from clip_amd import Config, Solver
+from tests import cases
+from tests.torch_step import TorchStep, interior_phi0, perturbed
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+def setup(n, radius, seed=0):
+    """Drop with a seeded perturbation: 1e-3 N(0,1) smoothed by one Laplacian
+    pass, confined to the interface band by 4 phi (1 - phi) (white noise in
+    the bulk phases diverges at density ratio 1000)."""
+    cfg = Config.from_string(cases.drop2d(n=n, radius=radius))
+    phi0 = interior_phi0(cfg)
+    return cfg, perturbed(phi0, seed)
+
+
+def target_of(cfg, phi0, nsteps, precision="f64"):
+    s = Solver(cfg.with_params(sigma=1.2 * cfg.sigma, mobility=0.8 * cfg.mobility), precision)
+    return s.adjoint_forward(phi0, nsteps, nsteps)
+
+
+@pytest.mark.parametrize("nfloor", [0.0, 0.1])
+def test_adjoint_matches_torch_tape(nfloor):
+    cfg, phi0 = setup(32, 8)
+    T, K = 20, 6
+    print("adjoint vs tape: drop2d 32^2", T, "steps, kept every", K, "normal floor", nfloor, flush=True)
+    target = target_of(cfg, phi0, T)
+    s = Solver(cfg, "f64")
+    phiT = s.adjoint_forward(phi0, T, K)
+    g0, gs, gm = s.adjoint_backward(phiT - target, nfloor)
+
+    ts = TorchStep(cfg, nfloor)
+    p0 = torch.tensor(phi0, requires_grad=True)
+    sg = torch.tensor(cfg.sigma, dtype=torch.float64, requires_grad=True)
+    mb = torch.tensor(cfg.mobility, dtype=torch.float64, requires_grad=True)
+    F, G, U, Phi = ts.run(p0, sg, mb, T)
+    assert np.abs(Phi.detach().numpy() - phiT).max() <= 1e-12
+    loss = 0.5 * ((Phi - torch.tensor(target)) ** 2).sum()
+    loss.backward()
+    e0 = rel(g0, p0.grad.numpy())
+    es = abs(gs - sg.grad.item()) / abs(sg.grad.item())
+    em = abs(gm - mb.grad.item()) / abs(mb.grad.item())
+    print(f"rel err phi0 {e0:.2e} sigma {es:.2e} mobility {em:.2e}", flush=True)
+    assert e0 <= 1e-9 and es <= 1e-9 and em <= 1e-9
+
+
+def loss_of(cfg, phi0, target, T, sigma=None, mobility=None):
+    s = Solver(cfg.with_params(sigma=sigma, mobility=mobility), "f64")
+    phiT = s.adjoint_forward(phi0, T, T)
+    return 0.5 * np.sum((phiT - target) ** 2)
+
+
+def test_adjoint_against_central_differences():
+    cfg, phi0 = setup(64, 16)
+    T, K = 100, 25
+    print("adjoint vs FD: drop2d 64^2", T, "steps", flush=True)
+    target = target_of(cfg, phi0, T)
+    s = Solver(cfg, "f64")
+    phiT = s.adjoint_forward(phi0, T, K)
+    g0, gs, gm = s.adjoint_backward(phiT - target)
+    rng = np.random.default_rng(1)
+    # 16 entries in the interface band.  On the plateaus (phi ~ 0 or 1) the
+    # normal's response to any perturbation above rounding is nonlinear, so a
+    # central difference does not measure the derivative there.
+    band = np.argwhere((phi0 > 0.05) & (phi0 < 0.95))
+    picks = band[rng.choice(len(band), 16, replace=False)].tolist()
+    worst = 0.0
+    for (i, j) in picks:
+        h = 1e-6 * abs(phi0[i, j])
+        pp, pm = phi0.copy(), phi0.copy()
+        pp[i, j] += h
+        pm[i, j] -= h
+        fd = (loss_of(cfg, pp, target, T) - loss_of(cfg, pm, target, T)) / (2 * h)
+        err = abs(fd - g0[i, j]) / max(abs(fd), 1e-3 * np.abs(g0).max())
+        worst = max(worst, err)
+        assert err <= 1e-4, ((i, j), fd, g0[i, j])
+    h = 1e-6 * cfg.sigma
+    fds = (loss_of(cfg, phi0, target, T, sigma=cfg.sigma + h) - loss_of(cfg, phi0, target, T, sigma=cfg.sigma - h)) / (2 * h)
+    h = 1e-6 * cfg.mobility
+    fdm = (loss_of(cfg, phi0, target, T, mobility=cfg.mobility + h)
+           - loss_of(cfg, phi0, target, T, mobility=cfg.mobility - h)) / (2 * h)
+    print(f"phi0 worst {worst:.2e}  sigma fd {fds:.6e} adj {gs:.6e}  mobility fd {fdm:.6e} adj {gm:.6e}", flush=True)
+    assert abs(fds - gs) <= 1e-4 * abs(fds)
+    assert abs(fdm - gm) <= 1e-4 * abs(fdm)
+
+
+def test_adjoint_3d_against_central_differences():
+    cfg = Config.from_string(cases.bubble3d(n=12, radius=3.5, rho_ratio=10, mu_ratio=10))
+    from tests.oracle_lib import Oracle
+    phi0 = Oracle(cfg).init().interior("phi")[..., 0]
+    T = 20
+    print("adjoint vs FD: bubble3d 12^3", T, "steps", flush=True)
+    tgt = Solver(cfg.with_params(sigma=1.2 * cfg.sigma), "f64").adjoint_forward(phi0, T, T)
+    s = Solver(cfg, "f64")
+    phiT = s.adjoint_forward(phi0, T, 7)
+    g0, gs, gm = s.adjoint_backward(phiT - tgt)
+
+    def L(p, sigma=None, mobility=None):
+        q = Solver(cfg.with_params(sigma=sigma, mobility=mobility), "f64").adjoint_forward(p, T, T)
+        return 0.5 * np.sum((q - tgt) ** 2)
+
+    for idx in [(6, 3, 6), (3, 6, 7), (8, 8, 5)]:
+        h = 1e-6 * max(abs(phi0[idx]), 1e-2)
+        pp, pm = phi0.copy(), phi0.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (L(pp) - L(pm)) / (2 * h)
+        assert abs(fd - g0[idx]) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(g0).max()), (idx, fd, g0[idx])
+    h = 1e-6 * cfg.sigma
+    fds = (L(phi0, sigma=cfg.sigma + h) - L(phi0, sigma=cfg.sigma - h)) / (2 * h)
+    h = 1e-6 * cfg.mobility
+    fdm = (L(phi0, mobility=cfg.mobility + h) - L(phi0, mobility=cfg.mobility - h)) / (2 * h)
+    print(f"3d sigma fd {fds:.6e} adj {gs:.6e}  mobility fd {fdm:.6e} adj {gm:.6e}", flush=True)
+    assert abs(fds - gs) <= 1e-4 * abs(fds)
+    assert abs(fdm - gm) <= 1e-4 * abs(fdm)
+
+
+@pytest.mark.parametrize("n,radius", [(64, 16)])
+def test_adjoint_fp32_against_fp64(n, radius):
+    """100 steps with the normal floor 0.1: FP32 gradients track FP64 (the
+    FP32 trajectory itself departs from the FP64 one by ~1e-6, and the
+    reverse sweep amplifies the difference)."""
+    cfg, phi0 = setup(n, radius)
+    T, K = 100, 20
+    target = target_of(cfg, phi0, T)
+    out = {}
+    for prec in ("f64", "f32"):
+        s = Solver(cfg, prec)
+        phiT = s.adjoint_forward(phi0, T, K)
+        out[prec] = s.adjoint_backward(phiT - target, 0.1)
+        assert np.all(np.isfinite(out[prec][0]))
+    e0 = rel(out["f32"][0], out["f64"][0])
+    es = abs(out["f32"][1] - out["f64"][1]) / abs(out["f64"][1])
+    em = abs(out["f32"][2] - out["f64"][2]) / abs(out["f64"][2])
+    print(f"fp32 vs fp64 adjoint ({n}^2, {T} steps): phi0 {e0:.2e} sigma {es:.2e} mobility {em:.2e}", flush=True)
+    assert e0 <= 5e-2 and es <= 5e-2 and em <= 5e-2
+
+
+def test_autograd_function():
+    from clip_amd.autograd import coupled_steps
+    cfg, phi0 = setup(32, 8)
+    T = 12
+    target = torch.tensor(target_of(cfg, phi0, T))
+    p0 = torch.tensor(phi0, requires_grad=True)
+    sg = torch.tensor(cfg.sigma, dtype=torch.float64, requires_grad=True)
+    mb = torch.tensor(cfg.mobility, dtype=torch.float64, requires_grad=True)
+    phiT = coupled_steps(p0, sg, mb, cfg, T, 5, "f64")
+    loss = 0.5 * ((phiT - target) ** 2).sum()
+    loss.backward()
+    s = Solver(cfg, "f64")
+    ref = s.adjoint_forward(phi0, T, 5)
+    g0, gs, gm = s.adjoint_backward(ref - target.numpy(), 0.1)
+    assert np.array_equal(p0.grad.numpy(), g0)
+    assert sg.grad.item() == gs and mb.grad.item() == gm
+
+
+def test_adjoint_rejects_non_periodic_and_mixed():
+    from clip_amd import ClipError
+    cfg = Config.from_string(cases.rti2d(16, 32))
+    with pytest.raises(ClipError):
+        Solver(cfg, "f64").adjoint_forward(np.zeros((16, 32)), 2, 1)
+    cfg, phi0 = setup(16, 4)
+    with pytest.raises(ClipError):
+        Solver(cfg, "mixed_f16").adjoint_forward(phi0, 2, 1)
