@@ -446,6 +446,9 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
 }
 
 // ------------------------------------------------------------ node selection
+// Threads per block of the step kernels; the bulk launches shape them as
+// bx x by with bx = 256 where rows allow it, so that each block reads whole
+// rows and a shifted pull straddles one extra 128-byte line per row only.
 constexpr int BX = 64, BY = 4;
 
 // Thread -> node of the bulk region: the interior box shrunk by one node
@@ -455,8 +458,8 @@ template <class L>
 __device__ __forceinline__ bool node_of_thread(const DevGrid& g, const DevBC& bc, int inset, int c[3]) {
     const int ix = inset && !bc.both_periodic[0], iy = inset && !bc.both_periodic[1];
     const int iz = (L::D == 3) ? (inset && !bc.both_periodic[2]) : 0;
-    c[0] = blockIdx.x * BX + threadIdx.x + g.dmin[0] + ix;
-    c[1] = blockIdx.y * BY + threadIdx.y + g.dmin[1] + iy;
+    c[0] = blockIdx.x * blockDim.x + threadIdx.x + g.dmin[0] + ix;
+    c[1] = blockIdx.y * blockDim.y + threadIdx.y + g.dmin[1] + iy;
     c[2] = blockIdx.z + g.dmin[2] + iz;
     return c[0] <= g.dmax[0] - ix && c[1] <= g.dmax[1] - iy;
 }

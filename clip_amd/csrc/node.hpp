@@ -191,11 +191,31 @@ CLIP_HD void force_of(const NodeScalars<T>& z, T p, const T dc[3], const T Pp[3]
     }
 }
 
+// |dc|.  In FP32, dc . dc underflows for |dc| below ~1e-19 (far-field tails
+// of the tanh profile), which would leave n = dc / (|dc| + 1e-32) far from
+// unit length; the magnitude is then taken from dc scaled by its largest
+// component.
+template <class L, class T>
+CLIP_HD T magnitude(const T dc[3]) {
+    if constexpr (sizeof(T) == 4) {
+        T m = fabs(dc[0]) > fabs(dc[1]) ? fabs(dc[0]) : fabs(dc[1]);
+        if (L::D == 3 && fabs(dc[2]) > m) m = fabs(dc[2]);
+        if (!(m > T(0))) return T(0);
+        if (m < T(1e-37)) return m;  // 1/m would overflow; within sqrt(D) of |dc|
+        const T r = T(1) / m;
+        T sq = (dc[0] * r) * (dc[0] * r) + (dc[1] * r) * (dc[1] * r);
+        if (L::D == 3) sq += (dc[2] * r) * (dc[2] * r);
+        return m * sqrt(sq);
+    } else {
+        T sq = dc[0] * dc[0] + dc[1] * dc[1];
+        if (L::D == 3) sq += dc[2] * dc[2];
+        return sqrt(sq);
+    }
+}
+
 template <class L, class T>
 CLIP_HD void normal_of(const T dc[3], T nr[3], T& rt, T& mag) {
-    T sq = dc[0] * dc[0] + dc[1] * dc[1];
-    if (L::D == 3) sq += dc[2] * dc[2];
-    mag = sqrt(sq);
+    mag = magnitude<L>(dc);
     rt = T(1) / (mag + T(1e-32));
 #pragma unroll
     for (int d = 0; d < 3; ++d) nr[d] = (d < L::D) ? dc[d] * rt : T(0);
@@ -263,7 +283,7 @@ CLIP_HD void normal_reverse(const T dc[3], T rt, T mag, T c0, T W, T nfloor, con
     for (int d = 0; d < D; ++d) nl += lnr[d] * dc[d] * rt;
     // 1/|dc| overflows FP32 for subnormal |dc|: drop the (bounded) second
     // term there
-    const T im = (mag > T(1e-30)) ? T(1) / mag : T(0);
+    const T im = (mag > T(1e-37)) ? T(1) / mag : T(0);
     const T k = sc * rt;
 #pragma unroll
     for (int d = 0; d < D; ++d) ldc[d] += k * (lnr[d] - nl * dc[d] * im);

@@ -48,6 +48,7 @@ struct SolverImpl : Impl {
     unsigned* red_bad = nullptr;
     double* red_out = nullptr;
     int cur = 1, ucur = 0, pcur = 0, phc = 0;
+    unsigned tbx = BX, tby = BY;  // block shape of the bulk launches (tbx * tby = BX * BY)
     bool old_ref_zero = false;  // pop[1-cur] encoded without phi reference (reference restart)
     bool timing_on = false;
     hipEvent_t ev[8] = {};
@@ -85,6 +86,11 @@ struct SolverImpl : Impl {
         }
         for (int a = 0; a < L::D; ++a)
             if (g.dmax[a] - g.dmin[a] + 1 < 3) throw ConfigError("the interior needs at least 3 nodes per axis");
+        {
+            const int nx = g.dmax[0] - g.dmin[0] + 1;
+            tbx = nx > 128 ? 256u : (nx > 64 ? 128u : 64u);
+            tby = (unsigned)(BX * BY) / tbx;
+        }
         g.sy = g.ext[0];
         g.sz = (long long)g.ext[0] * g.ext[1];
         g.nodes = g.sz * g.ext[2];
@@ -231,9 +237,10 @@ struct SolverImpl : Impl {
             n[a] = g.dmax[a] - g.dmin[a] + 1 - 2 * in;
         }
         if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return dim3(0, 0, 0);
-        return dim3((n[0] + BX - 1) / BX, (n[1] + BY - 1) / BY, n[2]);
+        return dim3((n[0] + tbx - 1) / tbx, (n[1] + tby - 1) / tby, n[2]);
     }
     dim3 grid_interior() const { return grid_box(0); }
+    dim3 block_shape() const { return dim3(tbx, tby); }
     long long shell_nodes() const {
         const ShellDesc sd = make_shell<L>(g, bc);
         long long t = 0;
@@ -251,13 +258,13 @@ struct SolverImpl : Impl {
     template <int MODE>
     void launch_fused_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
-        if (gb.x) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, dim3(BX, BY), 0, stream>>>(a);
+        if (gb.x) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a);
         if (shell_blocks()) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
         launch_check("k_fused");
     }
     void launch_phase_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
-        if (gb.x) k_phase<L, ST, CT, SP, false, false><<<gb, dim3(BX, BY), 0, stream>>>(a);
+        if (gb.x) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
         if (shell_blocks()) k_phase<L, ST, CT, SP, false, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
         launch_check("k_phase");
     }
@@ -333,7 +340,7 @@ struct SolverImpl : Impl {
         }
         // First collision straight from the initial f_post = f (pop 0).
         StepArgs<ST, CT> c = args(0, 0, 1, phib[0], phib[0]);
-        k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(c);
+        k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(c);
         launch_check("k_fused");
         HIP_CHECK(hipStreamSynchronize(stream));
         cur = 1;
@@ -531,7 +538,7 @@ struct SolverImpl : Impl {
         a.ucol = u[ucur];
         a.uout = nullptr;
         if (direct_old) {
-            k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
+            k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
             launch_check("k_fused");
         } else {
             launch_fused_pull<MODE_COLLIDE>(a);
@@ -639,12 +646,12 @@ struct SolverImpl : Impl {
         std::vector<double> zero((size_t)g.nodes * L::D, 0.0);
         upload(u[0], zero);
         StepArgs<ST, CT> a = args(0, 0, 1, phib[1], nullptr);
-        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
+        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
         launch_check("k_phase");
         // ghost ring from the resumed phi itself
         HIP_CHECK(hipMemcpyAsync(phib[0], phib[1], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice, stream));
         ghost_refresh(phib[0], phib[1]);
-        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), dim3(BX, BY), 0, stream>>>(a);
+        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
         launch_check("k_fused");
         k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(a, true, scratch, p);
         launch_check("k_export_post");

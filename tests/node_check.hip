@@ -7,8 +7,9 @@ using namespace clipamd;
 
 namespace {
 
-Phys<double> unpack(const double* a) {
-    Phys<double> ph;
+template <class T = double>
+Phys<T> unpack(const double* a) {
+    Phys<T> ph;
     ph.rho_h = a[0];
     ph.rho_l = a[1];
     ph.tau_h = a[2];
@@ -22,6 +23,37 @@ Phys<double> unpack(const double* a) {
     ph.grav[2] = a[10];
     ph.case_type = (int)a[11];
     return ph;
+}
+
+// FP32 reverse of one node (inputs given in FP64, rounded to FP32).
+template <class L>
+void rev32(const double* phys, const double* fp, const double* gp, double c0, const double* dc, double lap,
+           const double* uo, const double* lfs, const double* lgs, const double* lu, double nfloor, double* lfp,
+           double* lgp, double* lin) {
+    const Phys<float> ph = unpack<float>(phys);
+    float f[L::Q], g[L::Q], a[L::Q], b[L::Q];
+    for (int q = 0; q < L::Q; ++q) {
+        f[q] = (float)fp[q];
+        g[q] = (float)gp[q];
+        a[q] = (float)lfs[q];
+        b[q] = (float)lgs[q];
+    }
+    float dcc[3] = {(float)dc[0], (float)dc[1], L::D == 3 ? (float)dc[2] : 0.f};
+    float uoo[3] = {(float)uo[0], (float)uo[1], L::D == 3 ? (float)uo[2] : 0.f};
+    float luu[3] = {(float)lu[0], (float)lu[1], L::D == 3 ? (float)lu[2] : 0.f};
+    NodeAdj<L, float> o{};
+    node_reverse<L, float>(ph, f, g, (float)c0, dcc, (float)lap, uoo, a, b, luu, o, (float)nfloor);
+    for (int q = 0; q < L::Q; ++q) {
+        lfp[q] = o.lfp[q];
+        lgp[q] = o.lgp[q];
+    }
+    lin[0] = o.lc0;
+    for (int d = 0; d < 3; ++d) lin[1 + d] = o.ldc[d];
+    lin[4] = o.llap;
+    for (int d = 0; d < 3; ++d) lin[5 + d] = o.luo[d];
+    lin[8] = o.lbeta;
+    lin[9] = o.lkappa;
+    lin[10] = o.lmob;
 }
 
 template <class L>
@@ -119,6 +151,13 @@ int node_rev(int dim, const double* phys, const double* fp, const double* gp, do
              double* lgp, double* lin) {
     if (dim == 2) rev<D2Q9>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, lfp, lgp, lin);
     else rev<D3Q19>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, lfp, lgp, lin);
+    return 0;
+}
+int node_rev32(int dim, const double* phys, const double* fp, const double* gp, double c0, const double* dc,
+               double lap, const double* uo, const double* lfs, const double* lgs, const double* lu, double nfloor,
+               double* lfp, double* lgp, double* lin) {
+    if (dim == 2) rev32<D2Q9>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, nfloor, lfp, lgp, lin);
+    else rev32<D3Q19>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, nfloor, lfp, lgp, lin);
     return 0;
 }
 int init_fwd(int dim, const double* phys, double sigma, double half, double c0, const double* dc, double lap,

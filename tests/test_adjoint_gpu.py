@@ -128,11 +128,12 @@ def test_adjoint_3d_against_central_differences():
     assert abs(fdm - gm) <= 1e-4 * abs(fdm)
 
 
-@pytest.mark.parametrize("n,radius", [(64, 16)])
-def test_adjoint_fp32_against_fp64(n, radius):
-    """100 steps with the normal floor 0.1: FP32 gradients track FP64 (the
-    FP32 trajectory itself departs from the FP64 one by ~1e-6, and the
-    reverse sweep amplifies the difference)."""
+@pytest.mark.parametrize("n,radius,tol_phi,tol_par", [(64, 16, 5e-2, 5e-2), (256, 50, 0.25, 0.1)])
+def test_adjoint_fp32_against_fp64(n, radius, tol_phi, tol_par):
+    """100 steps with the normal floor 0.1: FP32 gradients track FP64.  The
+    FP32 trajectory departs from the FP64 one (1e-6 relative after 100
+    steps) and the reverse sweep amplifies the difference, more so for the
+    larger drop: measured 1.8e-2 (64^2) and 1.3e-1 (256^2) on dL/dphi0."""
     cfg, phi0 = setup(n, radius)
     T, K = 100, 20
     target = target_of(cfg, phi0, T)
@@ -146,7 +147,7 @@ def test_adjoint_fp32_against_fp64(n, radius):
     es = abs(out["f32"][1] - out["f64"][1]) / abs(out["f64"][1])
     em = abs(out["f32"][2] - out["f64"][2]) / abs(out["f64"][2])
     print(f"fp32 vs fp64 adjoint ({n}^2, {T} steps): phi0 {e0:.2e} sigma {es:.2e} mobility {em:.2e}", flush=True)
-    assert e0 <= 5e-2 and es <= 5e-2 and em <= 5e-2
+    assert e0 <= tol_phi and es <= tol_par and em <= tol_par
 
 
 def test_autograd_function():
