@@ -149,6 +149,23 @@ __device__ __forceinline__ CT gref(const CT* phi_ref, long long node) {
     else return CT(0);
 }
 
+// Reference node of the stored population q of interior node s: the node
+// that pulls it next, s + e_q, wrapped along axes periodic on both sides.
+// The mixed policy stores g relative to w_q phi there, so a bulk node
+// decodes all Q pulled values with its own phi (one load).
+template <class L>
+__device__ __forceinline__ long long ref_index(const DevGrid& g, const DevBC& bc, const int s[3], int q) {
+    int d[3] = {s[0] + L::ex(q), s[1] + L::ey(q), s[2] + L::ez(q)};
+#pragma unroll
+    for (int a = 0; a < L::D; ++a) {
+        if (!bc.both_periodic[a]) continue;
+        const int span = g.dmax[a] - g.dmin[a] + 1;
+        if (d[a] < g.dmin[a]) d[a] += span;
+        else if (d[a] > g.dmax[a]) d[a] -= span;
+    }
+    return g.idx(d[0], d[1], d[2]);
+}
+
 // ------------------------------------------------------------ pull with fix-ups
 // Post-streaming population q at interior node c, reproducing the reference's
 // streaming + boundary sequence:
@@ -168,7 +185,7 @@ struct Puller {
 
     __device__ CT load(int q, const int s[3]) const {
         const long long node = g.idx(s[0], s[1], s[2]);
-        const CT r = (GREF && g.interior(s, L::D)) ? gref<SP>(phi_ref, node) : CT(0);
+        const CT r = (GREF && g.interior(s, L::D)) ? gref<SP>(phi_ref, ref_index<L>(g, bc, s, q)) : CT(0);
         return SP::ld(A, (long long)q * g.nodes + node, q, r);
     }
 
@@ -390,12 +407,17 @@ __device__ __forceinline__ void body_force(const Phys<CT>& ph, CT rho, CT fg[3])
 // in lexicographic (x offset, y offset, z offset) order.
 template <class L, class CT>
 __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const DevGrid& g, long long n, CT& c0,
-                                            CT dc[3], CT& lap) {
+                                            CT dc[3], CT& lap, CT* nbq = nullptr) {
     c0 = phi[n];
     if (L::D == 2) {
         const long long sy = g.sy;
         const CT xp = phi[n + 1], xm = phi[n - 1], yp = phi[n + sy], ym = phi[n - sy];
         const CT pp = phi[n + 1 + sy], pm = phi[n + 1 - sy], mp = phi[n - 1 + sy], mm = phi[n - 1 - sy];
+        if (nbq) {
+            const CT v[3][3] = {{mm, xm, mp}, {ym, c0, yp}, {pm, xp, pp}};
+#pragma unroll
+            for (int q = 0; q < L::Q; ++q) nbq[q] = v[1 + L::ex(q)][1 + L::ey(q)];
+        }
         dc[0] = cdiv<CT>(xp - xm, 3.0) + cdiv<CT>((pm + pp) - (mm + mp), 12.0);
         dc[1] = cdiv<CT>(yp - ym, 3.0) + cdiv<CT>((mp + pp) - (mm + pm), 12.0);
         dc[2] = CT(0);
@@ -411,6 +433,10 @@ __device__ __forceinline__ void phi_stencil(const CT* __restrict__ phi, const De
             for (int b = 0; b < 3; ++b)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) v[a][b][d] = phi[n + (a - 1) + (b - 1) * sy + (d - 1) * sz];
+        if (nbq) {
+#pragma unroll
+            for (int q = 0; q < L::Q; ++q) nbq[q] = v[1 + L::ex(q)][1 + L::ey(q)][1 + L::ez(q)];
+        }
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             CT plus[3] = {0, 0, 0}, minus[3] = {0, 0, 0};
@@ -560,15 +586,16 @@ __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
     const long long n = a.g.idx(c[0], c[1], c[2]);
     CT sum = CT(0);
     if (DIRECT) {
-        const CT r = gref<SP>(a.phi_ref, n);
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q, r);
+        for (int q = 0; q < L::Q; ++q)
+            sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q, gref<SP>(a.phi_ref, ref_index<L>(a.g, a.bc, c, q)));
     } else if (!SHELL) {
         const PullOffsets po = pull_offsets<L>(a.g, c);
+        const CT r = gref<SP>(a.phi_ref, n);  // every pulled g is referenced to this node
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) {
             const long long src = pull_src<L>(n, po, q);
-            sum += SP::ld(a.gin, (long long)q * a.g.nodes + src, q, gref<SP>(a.phi_ref, src));
+            sum += SP::ld(a.gin, (long long)q * a.g.nodes + src, q, r);
         }
     } else {
         Puller<L, ST, CT, SP, true> P{a.gin, a.g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
@@ -598,7 +625,29 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 
     // phi stencil -> grad phi, chemical potential
     CT c0, dc[3], lap;
-    phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap);
+    CT nbq[Q];  // phi at n + e_q (the stencil's own loads)
+    phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap, SP::uses_ref ? nbq : nullptr);
+    // reference of the stored g (mixed policy): phi of the node that pulls it
+    bool seam = false;  // next to a side of an axis periodic on both ends
+    if constexpr (SP::uses_ref) {
+#pragma unroll
+        for (int ax = 0; ax < L::D; ++ax)
+            seam |= a.bc.both_periodic[ax] && (c[ax] == g.dmin[ax] || c[ax] == g.dmax[ax]);
+    }
+    CT refq[SP::uses_ref ? Q : 1];
+    if constexpr (SP::uses_ref) {
+        if (!seam) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) refq[q] = nbq[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) refq[q] = a.phi[ref_index<L>(g, a.bc, c, q)];
+        }
+    }
+    auto gout_ref = [&](int q) -> CT {
+        if constexpr (!SP::uses_ref) return CT(0);
+        else return refq[q];
+    };
 
     PullOffsets po;
     if (!DIRECT && !SHELL) po = pull_offsets<L>(g, c);
@@ -621,11 +670,12 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         }
     }
     // g_post
+    const CT rn = (!DIRECT && !SHELL) ? gref<SP>(a.phi_ref, n) : CT(0);
     auto pull_g = [&](int q) -> CT {
-        if (DIRECT) return SP::ld(a.gin, (long long)q * g.nodes + n, q, gref<SP>(a.phi_ref, n));
+        if (DIRECT) return SP::ld(a.gin, (long long)q * g.nodes + n, q, gref<SP>(a.phi_ref, ref_index<L>(g, a.bc, c, q)));
         if (!SHELL) {
             const long long src = pull_src<L>(n, po, q);
-            return SP::ld(a.gin, (long long)q * g.nodes + src, q, gref<SP>(a.phi_ref, src));
+            return SP::ld(a.gin, (long long)q * g.nodes + src, q, rn);
         }
         Puller<L, ST, CT, SP, true> P{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
         return P.lvl3(c, q);
@@ -664,7 +714,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         }
         collide_node<L, CT>(
             z, p, Pf, c0, dc, u, gp,
-            [&](int q, CT v) { SP::st(a.gout, (long long)q * g.nodes + n, q, v, c0); },
+            [&](int q, CT v) { SP::st(a.gout, (long long)q * g.nodes + n, q, v, gout_ref(q)); },
             [&](int q, CT v) { SP::st(a.fout, (long long)q * g.nodes + n, q, v, CT(0)); });
         return;
     } else {
@@ -735,7 +785,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
                 CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
                 if (L::D == 3) en += CT(L::ez(q)) * nr[2];
                 const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);
-                SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc, c0);
+                SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc, CT(0));
             }
         }
 
@@ -872,12 +922,16 @@ __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
     if (interior) {
         CT cc, dc[3], lap;
         phi_stencil<L, CT>(a.phi, g, n, cc, dc, lap);
-        CT sq = dc[0] * dc[0] + dc[1] * dc[1];
-        if (L::D == 3) sq += dc[2] * dc[2];
-        const CT t = sqrt(sq) + CT(1e-32);
-        for (int d = 0; d < L::D; ++d) nr[d] = dc[d] / t;
+        if constexpr (sizeof(CT) == 4) {
+            CT rt, mag;
+            normal_of<L>(dc, nr, rt, mag);
+        } else {
+            CT sq = dc[0] * dc[0] + dc[1] * dc[1];
+            if (L::D == 3) sq += dc[2] * dc[2];
+            const CT t = sqrt(sq) + CT(1e-32);
+            for (int d = 0; d < L::D; ++d) nr[d] = dc[d] / t;
+        }
     }
-    const CT gr = interior ? c0 : CT(0);
     const CT u0[3] = {0, 0, 0};
     const CT hw = (CT(1) - CT(4) * ((c0 - CT(0.5)) * (c0 - CT(0.5)))) / ph.W;
 #pragma unroll
@@ -891,6 +945,7 @@ __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
         const long long i = (long long)q * g.nodes + n;
         SP::st(a.fout, i, q, fv, CT(0));
         SP::st(f2, i, q, fv, CT(0));
+        const CT gr = (SP::uses_ref && interior) ? a.phi[ref_index<L>(g, a.bc, c, q)] : CT(0);
         SP::st(a.gout, i, q, gv, gr);
         SP::st(g2, i, q, gv, gr);
         if (a.bc.is_fc && interior) {
@@ -981,7 +1036,7 @@ __global__ void k_export_post(StepArgs<ST, CT> a, bool is_f, CT* out, CT* pout) 
     for (int q = 0; q < L::Q; ++q) {
         CT v;
         if (!interior) v = SP::ld(src, (long long)q * g.nodes + n, q, CT(0));
-        else if (DIRECT) v = SP::ld(src, (long long)q * g.nodes + n, q, gref<SP>(ref, n));
+        else if (DIRECT) v = SP::ld(src, (long long)q * g.nodes + n, q, gref<SP>(ref, ref_index<L>(g, a.bc, c, q)));
         else if (is_f) v = Puller<L, ST, CT, SP, false>{src, g, a.bc, a.uin, prev, ref}.lvl3(c, q);
         else v = Puller<L, ST, CT, SP, true>{src, g, a.bc, a.uin, prev, ref}.lvl3(c, q);
         out[(long long)q * g.nodes + n] = v;
@@ -992,26 +1047,26 @@ __global__ void k_export_post(StepArgs<ST, CT> a, bool is_f, CT* out, CT* pout) 
 
 // Stored populations <-> dense CT array (ref = phi at interior nodes for g,
 // nullptr for f).
-template <int Q, class ST, class CT, class SP>
-__global__ void k_export_raw(const ST* src, CT* out, DevGrid g, int D, const CT* ref) {
+template <class L, class ST, class CT, class SP>
+__global__ void k_export_raw(const ST* src, CT* out, DevGrid g, DevBC bc, const CT* ref) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.nodes * Q) return;
+    if (t >= g.nodes * L::Q) return;
     const long long n = t % g.nodes;
     const int q = (int)(t / g.nodes);
     const int c[3] = {(int)(n % g.ext[0]), (int)((n / g.ext[0]) % g.ext[1]),
                       (int)(n / ((long long)g.ext[0] * g.ext[1]))};
-    out[t] = SP::ld(src, t, q, g.interior(c, D) ? gref<SP>(ref, n) : CT(0));
+    out[t] = SP::ld(src, t, q, g.interior(c, L::D) ? gref<SP>(ref, ref_index<L>(g, bc, c, q)) : CT(0));
 }
 
-template <int Q, class ST, class CT, class SP>
-__global__ void k_import_raw(ST* dst, const CT* in, DevGrid g, int D, const CT* ref) {
+template <class L, class ST, class CT, class SP>
+__global__ void k_import_raw(ST* dst, const CT* in, DevGrid g, DevBC bc, const CT* ref) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.nodes * Q) return;
+    if (t >= g.nodes * L::Q) return;
     const long long n = t % g.nodes;
     const int q = (int)(t / g.nodes);
     const int c[3] = {(int)(n % g.ext[0]), (int)((n / g.ext[0]) % g.ext[1]),
                       (int)(n / ((long long)g.ext[0] * g.ext[1]))};
-    SP::st(dst, t, q, in[t], g.interior(c, D) ? gref<SP>(ref, n) : CT(0));
+    SP::st(dst, t, q, in[t], g.interior(c, L::D) ? gref<SP>(ref, ref_index<L>(g, bc, c, q)) : CT(0));
 }
 
 // mu, grad phi and normal at interior nodes (ghosts left at 0).

@@ -461,8 +461,8 @@ struct SolverImpl : Impl {
                 const bool is_f = field == CLIP_FIELD_F;
                 const ST* src = is_f ? fbuf[old] : gbuf[old];
                 const long long count = (long long)nodes * QL;
-                k_export_raw<QL, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(
-                    src, scratch, g, L::D, is_f ? nullptr : old_ref());
+                k_export_raw<L, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(
+                    src, scratch, g, bc, is_f ? nullptr : old_ref());
                 launch_check("k_export_raw");
                 download(scratch, count, out);
                 const auto& keep = is_f ? direct_f_interior : direct_g_interior;
@@ -512,7 +512,7 @@ struct SolverImpl : Impl {
         const long long count = (long long)in.size();
         upload(scratch, in);
         const int tb = 256;
-        k_import_raw<QL, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(dev, scratch, g, L::D, ref);
+        k_import_raw<L, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(dev, scratch, g, bc, ref);
         launch_check("k_import_raw");
         HIP_CHECK(hipStreamSynchronize(stream));
     }
