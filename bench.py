@@ -83,6 +83,26 @@ def cpu_baseline(text, steps_cap, seconds_budget):
                       f"OpenMP over {cores} threads, {dt:.1f} s"}
 
 
+def copy_bandwidth(mib=2048, reps=10):
+    """Device-to-device copy (read + write) GB/s on torch's stream: the
+    practical HBM ceiling the step kernels are compared with."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def adjoint_measure(precision, n=256, radius=50, steps=500, every=50):
     """Reverse-mode cost on the 2D drop (D2Q9, n^2, R = radius, density ratio
     1000): loss = 1/2 |phi_T - phi_target|^2 with the target from a run with
@@ -198,6 +218,7 @@ def main():
     health = sol.check_health()
     sol.close()
     del sol
+    copy_gbs = copy_bandwidth()
 
     line = {
         "metric": "MLUPS + % HBM peak (D3Q19 coupled step) FP64 vs mixed; adjoint/forward time",
@@ -219,7 +240,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic("k_fused", args.workload, args.precision),
                      "kernel": "k_fused", "kernel_ms": round(fused_ms, 4),
-                     "alg_bytes_per_launch": fused_bytes},
+                     "alg_bytes_per_launch": fused_bytes,
+                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "step_hbm_frac": round(mlups * 1e6 * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms_per_step": {k: round(v / tsteps, 4) for k, v in tm.items()},
         "phi_sum": health,
