@@ -45,6 +45,8 @@ def workload_text(name):
         return cases.rti2d(256, 1024)
     if name == "drop2d128":
         return cases.drop2d(n=128, radius=25)
+    if name == "jet3d192x192x768":
+        return cases.jet3d_c5()
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -81,6 +83,34 @@ def cpu_baseline(text, steps_cap, seconds_budget):
     return {"value": interior * done / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": f"{done} step(s) of the same workload after initialisation, FP64 oracle, "
                       f"OpenMP over {cores} threads, {dt:.1f} s"}
+
+
+def measure(workload, precision, steps=50, warmup=5):
+    """MLUPS and the k_fused roofline of one more workload/precision."""
+    import torch
+    from clip_amd import Config, Solver
+    cfg = Config.from_string(workload_text(workload))
+    sol = Solver(cfg, precision).init_fresh()
+    inf = sol.info()
+    sol.step(warmup)
+    sol.synchronize()
+    t0 = time.perf_counter()
+    sol.step(steps)
+    sol.synchronize()
+    dt = time.perf_counter() - t0
+    sol.enable_timing(True)
+    sol.step(min(steps, 20))
+    tm = sol.last_timing()
+    sol.close()
+    torch.cuda.synchronize()
+    n = int(inf.interior)
+    k = min(steps, 20)
+    fused_ms = tm["fused_ms"] / k
+    alg = n * (4 * inf.q * pop_bytes(precision) + (2 * inf.dim + 1) * mac_bytes(precision))
+    return {"workload": workload, "precision": precision, "interior_nodes": n, "lattice": f"D{inf.dim}Q{inf.q}",
+            "MLUPS": round(n * steps / dt / 1e6, 2), "ms_per_step": round(dt * 1e3 / steps, 4),
+            "kernel_ms_per_step": {kk: round(v / k, 4) for kk, v in tm.items()},
+            "k_fused_hbm_frac": round(alg / (fused_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def copy_bandwidth(mib=2048, reps=10):
@@ -162,6 +192,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode measurement")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and jet lines")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run an elementwise FP32 copy (read + write) of this many MiB five times; "
                          "its rocprofv3 FETCH_SIZE / WRITE_SIZE calibrate the traffic counters")
@@ -246,6 +277,10 @@ def main():
         "kernel_ms_per_step": {k: round(v / tsteps, 4) for k, v in tm.items()},
         "phi_sum": health,
     }
+    if not args.no_extra:
+        other = "mixed_f16" if args.precision == "f64" else "f64"
+        line["extra"] = [measure(args.workload, other), measure("jet3d192x192x768", "mixed_f16"),
+                         measure("jet3d192x192x768", "f64")]
     if not args.no_adjoint:
         line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64")]
     if not args.no_cpu_baseline:

@@ -107,3 +107,26 @@ def jet3d(n=(24, 32, 24), radius=5.0, U=0.02, rho_ratio=4, mu_ratio=2, W=4, inle
 def box2d(sides, n=24, ny=32, case="drop", radius=6.0):
     """2D drop resting mid-box with arbitrary side conditions and gravity -y."""
     return drop2d(n=n, ny=ny, radius=radius, rho_ratio=10, mu_ratio=10, gravity=(0, -1e-5, 0), sides=sides)
+
+
+def jet3d_c5(nx=192, ny=192, nz=768, radius=16.0, U=0.05, L=32.0, rho_ratio=100, mu_ratio=10, Re=500, We=50,
+             modes=8, seed=0, W=4):
+    """3D jet: inlet on z- through a disc of the given radius centred in x, y
+    (sphere geometry on the inlet plane), u_z = U (1 + sum_m A_m cos(m theta +
+    phase_m)) with A_m ~ 1e-2 N(0,1), phase_m ~ U(0, 2 pi) (seeded), convective
+    outflow on z+, periodic x and y.  Interior nx x ny x nz (N includes the
+    ghost ring)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    amp = 1e-2 * rng.standard_normal(modes)
+    ph = rng.uniform(0.0, 2 * np.pi, modes)
+    N = (nx + 2, ny + 2, nz + 2)
+    cen = ((N[0] - 1) / 2.0, (N[1] - 1) / 2.0, 0)
+    sides = [("x-", "periodic"), ("x+", "periodic"), ("y-", "periodic"), ("y+", "periodic"),
+             ("z-", "velocity", [0, 0, U]), ("z+", "free convect")]
+    return (f"case = jet\nfinalStep = 3000\nN = [{N[0]}, {N[1]}, {N[2]}]\nreferenceLength = {L}\n"
+            f"referenceVelocity = {U}\nWe = {We}\nRe = {Re}\ngravity = [0, 0, 0]\nmobility = 0.02\n"
+            f"interfaceWidth = {W}\nmuRatio = {mu_ratio}\nrhoRatio = {rho_ratio}\n"
+            f"inletModes = {modes}\ninletModeAmplitude = [{', '.join(repr(float(a)) for a in amp)}]\n"
+            f"inletModePhase = [{', '.join(repr(float(p)) for p in ph)}]\n\n"
+            + _geom("sphere", cen, radius) + "\n" + _bc(sides))

@@ -113,6 +113,15 @@ def test_jet3d_zminus_neumann_outlet():
     assert_fields(gpu, orc)
 
 
+def test_jet3d_modulated_inlet():
+    """The 3D jet workload's shape at small size: z- disc inlet with eight
+    azimuthal modes, convective outflow on z+, periodic x and y."""
+    cfg, gpu, orc = run_pair(cases.jet3d_c5(nx=20, ny=20, nz=32, radius=5.0, L=10.0, U=0.02), 80)
+    assert_fields(gpu, orc)
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL
+
+
 def test_box3d_walls():
     sides = [("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall"), ("z-", "periodic"), ("z+", "periodic")]
     cfg, gpu, orc = run_pair(cases.bubble3d(n=14, radius=3, rho_ratio=10, mu_ratio=10, sides=sides), 80)
