@@ -120,6 +120,11 @@ struct ShellDesc {
     int axes[3];
     long long count[3];
 };
+__host__ __device__ inline long long shell_total(const ShellDesc& sd) {
+    long long t = 0;
+    for (int k = 0; k < sd.na; ++k) t += sd.count[k];
+    return t;
+}
 
 template <class ST, class CT>
 struct StepArgs {
@@ -137,6 +142,8 @@ struct StepArgs {
     CT* uout;        // u of this macroscopic step
     const CT* ucol;  // u the collide-only mode collides with
     CT* p;           // pressure (read by the collide-only mode; written by exports)
+    CT* shell_f;     // boundary-layer f_post / g_post (Q x shell nodes), written by k_shell_pull
+    CT* shell_g;
     const CT* prev_f_in[6];  // outflow planes: f_post of the previous step
     CT* prev_f_out[6];
     const CT* prev_g_in[6];
@@ -183,13 +190,13 @@ struct Puller {
     const CT* const* prev;
     const CT* phi_ref;
 
-    __device__ CT load(int q, const int s[3]) const {
+    __device__ __forceinline__ CT load(int q, const int s[3]) const {
         const long long node = g.idx(s[0], s[1], s[2]);
         const CT r = (GREF && g.interior(s, L::D)) ? gref<SP>(phi_ref, ref_index<L>(g, bc, s, q)) : CT(0);
         return SP::ld(A, (long long)q * g.nodes + node, q, r);
     }
 
-    __device__ CT raw(const int c[3], int q) const {
+    __device__ __forceinline__ CT raw(const int c[3], int q) const {
         int s[3] = {c[0] - L::ex(q), c[1] - L::ey(q), c[2] - L::ez(q)};
 #pragma unroll
         for (int a = 0; a < L::D; ++a) {
@@ -219,7 +226,7 @@ struct Puller {
         return load(q, s);
     }
 
-    __device__ CT lvl1(const int c[3], int q) const {
+    __device__ __forceinline__ CT lvl1(const int c[3], int q) const {
         if (bc.is_wall) {
 #pragma unroll
             for (int side = 0; side < 2 * L::D; ++side) {
@@ -234,7 +241,7 @@ struct Puller {
         return raw(c, q);
     }
 
-    __device__ CT lvl2(const int c[3], int q) const {
+    __device__ __forceinline__ CT lvl2(const int c[3], int q) const {
         if (bc.is_fc) {
 #pragma unroll
             for (int side = 2 * L::D - 1; side >= 0; --side) {
@@ -253,7 +260,7 @@ struct Puller {
         return lvl1(c, q);
     }
 
-    __device__ CT lvl3(const int c[3], int q) const {
+    __device__ __forceinline__ CT lvl3(const int c[3], int q) const {
         if (bc.is_neumann) {
 #pragma unroll
             for (int side = 2 * L::D - 1; side >= 0; --side) {
@@ -512,23 +519,25 @@ __host__ __device__ inline ShellDesc make_shell(const DevGrid& g, const DevBC& b
 }
 
 template <class L>
-__device__ __forceinline__ bool shell_node(const DevGrid& g, const ShellDesc& sd, int c[3]) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ bool shell_node_at(const DevGrid& g, const ShellDesc& sd, long long t64, int c[3]) {
+    // 32-bit index arithmetic: a boundary layer holds far fewer than 2^32 nodes
+    unsigned t = (unsigned)t64;
     for (int k = 0; k < sd.na; ++k) {
-        if (t >= sd.count[k]) {
-            t -= sd.count[k];
+        const unsigned cnt = (unsigned)sd.count[k];
+        if (t >= cnt) {
+            t -= cnt;
             continue;
         }
         const int a = sd.axes[k];
-        const long long half = sd.count[k] / 2;
+        const unsigned half = cnt / 2;
         const bool hi = t >= half;
-        long long r = hi ? t - half : t;
+        unsigned r = hi ? t - half : t;
         for (int b = 0; b < 3; ++b) {
             if (b == a) continue;
             bool earlier = false;
             for (int m = 0; m < k; ++m) earlier |= sd.axes[m] == b;
             const int lo = g.dmin[b] + (earlier ? 1 : 0);
-            const long long size = (long long)(g.dmax[b] - g.dmin[b] + 1) - (earlier ? 2 : 0);
+            const unsigned size = (unsigned)(g.dmax[b] - g.dmin[b] + 1 - (earlier ? 2 : 0));
             c[b] = lo + (int)(r % size);
             r /= size;
         }
@@ -536,6 +545,11 @@ __device__ __forceinline__ bool shell_node(const DevGrid& g, const ShellDesc& sd
         return true;
     }
     return false;
+}
+
+template <class L>
+__device__ __forceinline__ bool shell_node(const DevGrid& g, const ShellDesc& sd, int c[3]) {
+    return shell_node_at<L>(g, sd, (long long)blockIdx.x * blockDim.x + threadIdx.x, c);
 }
 
 // Source index of population q pulled by a bulk node: n - e_q, wrapped along
@@ -605,6 +619,51 @@ __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
     a.phi[n] = sum;
 }
 
+// Boundary layer, first half of the step: post-streaming f and g of every
+// (shell node, q) pair with the boundary fix-ups (Puller), into a dense
+// scratch that the shell launch of k_fused reads, and the outflow planes of
+// the next step.  One thread per pair spreads the fix-up chains over the
+// chip; keeping them out of k_fused keeps that kernel within its registers.
+template <class L, class ST, class CT, class SP>
+__global__ __launch_bounds__(BX* BY) void k_shell_pull(StepArgs<ST, CT> a) {
+    const unsigned S = (unsigned)shell_total(a.shell);
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S * (unsigned)L::Q) return;
+    const unsigned t = i % S;
+    const int myq = (int)(i / S);
+    int c[3];
+    if (!shell_node_at<L>(a.g, a.shell, t, c)) return;
+    const DevGrid& g = a.g;
+    Puller<L, ST, CT, SP, false> Pf{a.fin, g, a.bc, a.uin, a.prev_f_in, a.phi_ref};
+    Puller<L, ST, CT, SP, true> Pg{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        if (q != myq) continue;
+        const CT fv = Pf.lvl3(c, q);
+        const CT gv = Pg.lvl3(c, q);
+        a.shell_f[(long long)q * S + t] = fv;
+        a.shell_g[(long long)q * S + t] = gv;
+        if (a.bc.is_fc) {
+            store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fv);
+            store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gv);
+        }
+    }
+}
+
+// phi^{n+1} = sum_q g_post at the boundary-layer nodes (q ascending, as
+// k_phase sums).
+template <class L, class ST, class CT>
+__global__ __launch_bounds__(BX* BY) void k_shell_phi(StepArgs<ST, CT> a) {
+    const long long S = shell_total(a.shell);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c[3];
+    if (t >= S || !shell_node_at<L>(a.g, a.shell, t, c)) return;
+    CT sum = CT(0);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) sum += a.shell_g[(long long)q * S + t];
+    a.phi[a.g.idx(c[0], c[1], c[2])] = sum;
+}
+
 enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 
 // Fused macroscopic + collision (kernelMacroscopicf NsAllen.cu:369-511,
@@ -617,6 +676,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     constexpr bool MOMENT_FORM = sizeof(CT) == 4;
     int c[3];
     if (!select_node<L, DIRECT, SHELL>(a.g, a.bc, a.shell, c)) return;
+    const long long shell_t = SHELL ? (long long)blockIdx.x * blockDim.x + threadIdx.x : 0;
     const DevGrid& g = a.g;
     const Phys<CT>& ph = a.ph;
     const long long n = g.idx(c[0], c[1], c[2]);
@@ -655,16 +715,15 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     // f_post
     CT fp[Q];
     {
-        Puller<L, ST, CT, SP, false> P{a.fin, g, a.bc, a.uin, a.prev_f_in, a.phi_ref};
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q, CT(0));
             else if (!SHELL)
                 fp[q] = SP::ld(a.fin, (long long)q * g.nodes + pull_src<L>(n, po, q), q, CT(0));
             else
-                fp[q] = P.lvl3(c, q);
+                fp[q] = a.shell_f[(long long)q * shell_total(a.shell) + shell_t];
         }
-        if (a.bc.is_fc && edge) {
+        if (DIRECT && a.bc.is_fc && edge) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fp[q]);
         }
@@ -677,8 +736,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
             const long long src = pull_src<L>(n, po, q);
             return SP::ld(a.gin, (long long)q * g.nodes + src, q, rn);
         }
-        Puller<L, ST, CT, SP, true> P{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
-        return P.lvl3(c, q);
+        return a.shell_g[(long long)q * shell_total(a.shell) + shell_t];
     };
 
     if constexpr (MOMENT_FORM) {
@@ -710,7 +768,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             gp[q] = pull_g(q);
-            if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp[q]);
+            if (DIRECT && a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp[q]);
         }
         collide_node<L, CT>(
             z, p, Pf, c0, dc, u, gp,
@@ -781,7 +839,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const CT gp = pull_g(q);
-                if (a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
+                if (DIRECT && a.bc.is_fc && edge) store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gp);
                 CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
                 if (L::D == 3) en += CT(L::ez(q)) * nr[2];
                 const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);

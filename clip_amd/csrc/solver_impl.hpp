@@ -44,6 +44,8 @@ struct SolverImpl : Impl {
     CT* prevg[2][6] = {};
     unsigned char* vflag[6] = {};
     CT* scratch = nullptr;        // export staging (max(Q, D) * nodes)
+    CT* shell_f = nullptr;        // boundary-layer f_post / g_post (Q * shell nodes)
+    CT* shell_g = nullptr;
     double* red_part = nullptr;   // reduction partials
     unsigned* red_bad = nullptr;
     double* red_out = nullptr;
@@ -87,9 +89,13 @@ struct SolverImpl : Impl {
         for (int a = 0; a < L::D; ++a)
             if (g.dmax[a] - g.dmin[a] + 1 < 3) throw ConfigError("the interior needs at least 3 nodes per axis");
         {
+            // rows split into equal chunks of at most 256 nodes, each rounded
+            // up to whole waves
             const int nx = g.dmax[0] - g.dmin[0] + 1;
-            tbx = nx > 128 ? 256u : (nx > 64 ? 128u : 64u);
-            tby = (unsigned)(BX * BY) / tbx;
+            const int chunks = (nx + BX * BY - 1) / (BX * BY);
+            const int per = (nx + chunks - 1) / chunks;
+            tbx = (unsigned)((per + 63) / 64 * 64);
+            tby = std::max(1u, (unsigned)(BX * BY) / tbx);
         }
         g.sy = g.ext[0];
         g.sz = (long long)g.ext[0] * g.ext[1];
@@ -137,6 +143,11 @@ struct SolverImpl : Impl {
         }
         p = alloc<CT>(nodes);
         scratch = alloc<CT>(nodes * std::max(QL, L::D));
+        if (shell_nodes() > 0) {
+            if (shell_nodes() * QL >= (1LL << 31)) throw ConfigError("boundary layer too large");
+            shell_f = alloc<CT>((size_t)shell_nodes() * QL);
+            shell_g = alloc<CT>((size_t)shell_nodes() * QL);
+        }
         red_part = alloc<double>(1024);
         red_bad = alloc<unsigned>(1);
         red_out = alloc<double>(1);
@@ -219,6 +230,8 @@ struct SolverImpl : Impl {
         a.uout = u[1 - uin];
         a.ucol = u[uin];
         a.p = p;
+        a.shell_f = shell_f;
+        a.shell_g = shell_g;
         for (int s = 0; s < 6; ++s) {
             a.prev_f_in[s] = prevf[prev_in][s];
             a.prev_f_out[s] = prevf[1 - prev_in][s];
@@ -262,11 +275,22 @@ struct SolverImpl : Impl {
         if (shell_blocks()) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
         launch_check("k_fused");
     }
+    // phi^{n+1} of the bulk (k_phase) and of the boundary layer (k_shell_pull,
+    // which also leaves the layer's f_post / g_post for k_fused).
     void launch_phase_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
         if (gb.x) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
-        if (shell_blocks()) k_phase<L, ST, CT, SP, false, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
         launch_check("k_phase");
+        if (shell_blocks()) {
+            launch_shell_pull(a);
+            k_shell_phi<L, ST, CT><<<shell_blocks(), BX * BY, 0, stream>>>(a);
+            launch_check("k_shell_phi");
+        }
+    }
+    void launch_shell_pull(const StepArgs<ST, CT>& a) {
+        const unsigned blocks = (unsigned)((shell_nodes() * QL + BX * BY - 1) / (BX * BY));
+        k_shell_pull<L, ST, CT, SP><<<blocks, BX * BY, 0, stream>>>(a);
+        launch_check("k_shell_pull");
     }
 
     // Ghost ring of phi_new from phi_old (Solver.cu:75-141, 550-921, 927-978).
@@ -541,6 +565,7 @@ struct SolverImpl : Impl {
             k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
             launch_check("k_fused");
         } else {
+            if (shell_blocks()) launch_shell_pull(a);
             launch_fused_pull<MODE_COLLIDE>(a);
         }
         HIP_CHECK(hipStreamSynchronize(stream));
