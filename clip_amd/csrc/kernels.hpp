@@ -301,17 +301,6 @@ __device__ inline void store_prev(const DevGrid& g, const DevBC& bc, CT* const* 
 }
 
 // ------------------------------------------------------------ node math
-// Quotient x / d from r = 1/d (r correctly rounded): q0 = x r, residual
-// x - q0 d exact by FMA, q = q0 + residual r (Markstein, IBM J. Res. Dev.
-// 34(1), 1990).  Equals the IEEE quotient for the operands of this path
-// (checked against plain division by the CPU tests), so the FP64 results stay
-// identical to the oracle's while each reuse of a divisor costs 3 operations.
-template <class T>
-__device__ __forceinline__ T div_rn(T x, T d, T r) {
-    const T q0 = x * r;
-    const T rem = __builtin_fma(-q0, d, x);
-    return __builtin_fma(rem, r, q0);
-}
 // Division by a compile-time constant: exact quotient in FP64, multiplication
 // by the reciprocal in FP32.
 template <class CT>

@@ -25,6 +25,19 @@
 
 namespace clipamd {
 
+// Quotient x / d from r = 1/d (r correctly rounded): q0 = x r, residual
+// x - q0 d exact by FMA, q = q0 + residual r (Markstein, IBM J. Res. Dev.
+// 34(1), 1990).  Equals the IEEE quotient for the operands of this path
+// (checked against plain division on random operands by the CPU tests), so
+// the FP64 results stay identical to the oracle's while each reuse of a
+// divisor costs 3 operations.
+template <class T>
+CLIP_HD T div_rn(T x, T d, T r) {
+    const T q0 = x * r;
+    const T rem = fma(-q0, d, x);
+    return fma(rem, r, q0);
+}
+
 template <class CT>
 struct Phys {
     CT rho_h, rho_l, tau_h, tau_l, beta, kappa, mobility, W;

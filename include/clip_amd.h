@@ -186,7 +186,8 @@ int clip_check_health(clip_solver* s, double* phi_sum);
 /* CheckPointer::save / load (CheckPointer.cu:53-102): CLIP's file set in
  * `dir` (dev_f.bin, dev_g.bin, dev_f_post.bin, dev_g_post.bin,
  * [dev_f_prev.bin, dev_g_prev.bin], timeInfo.bin, domainSize.bin, info.txt)
- * plus clip_amd_state.bin (u, phi with ghosts, outflow planes) so that a
+ * plus clip_amd_state.bin (phi of the last two steps with their ghost rings
+ * and u_old; the outflow planes go to dev_f_prev / dev_g_prev) so that a
  * resume is bit-exact.  Loading without the extension file reproduces the
  * reference's macroscopic() restart with u_old = 0. */
 int clip_checkpoint_save(clip_solver* s, const char* dir);

@@ -160,6 +160,16 @@ int node_rev32(int dim, const double* phys, const double* fp, const double* gp, 
     else rev32<D3Q19>(phys, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, nfloor, lfp, lgp, lin);
     return 0;
 }
+// x[i] / d through div_rn with r = 1/d (for the division tests).
+int div_rn_batch(const double* x, long long n, double d, double* out) {
+    const double r = 1.0 / d;
+    for (long long i = 0; i < n; ++i) out[i] = div_rn(x[i], d, r);
+    return 0;
+}
+int div_rn_pairs(const double* x, const double* d, long long n, double* out) {
+    for (long long i = 0; i < n; ++i) out[i] = div_rn(x[i], d[i], 1.0 / d[i]);
+    return 0;
+}
 int init_fwd(int dim, const double* phys, double sigma, double half, double c0, const double* dc, double lap,
              double* fs, double* gs) {
     if (dim == 2) init_fwd<D2Q9>(phys, sigma, half, c0, dc, lap, fs, gs);
