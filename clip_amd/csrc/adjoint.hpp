@@ -185,10 +185,19 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a) {
 }
 
 // Transpose of the pull: adjoint of f*, g* of S^n at interior node y.
+// acc[k] += sum over blocks of part[4 b + k], in block order.
+__device__ __forceinline__ void accumulate4(const double* part, int blocks, double* acc) {
+    if (threadIdx.x >= 4) return;
+    double s = 0.0;
+    for (int b = 0; b < blocks; ++b) s += part[4 * b + threadIdx.x];
+    acc[threadIdx.x] += s;
+}
+
 template <class L, class CT>
-__global__ __launch_bounds__(ADJ_TB) void k_adj_pull(AdjArgs<CT> a) {
+__global__ __launch_bounds__(ADJ_TB) void k_adj_pull(AdjArgs<CT> a, double* acc, int blocks) {
     constexpr int Q = L::Q, D = L::D;
     const DevGrid& g = a.g;
+    if (blockIdx.x == 0) accumulate4(a.part, blocks, acc);  // partials of k_adj_node
     int c[3];
     if (!interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
     const long long n = g.idx(c[0], c[1], c[2]);
@@ -239,10 +248,7 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_init(AdjArgs<CT> a) {
 // acc[k] += sum over blocks of part[4 b + k], in block order.
 template <class CT>
 __global__ void k_adj_accumulate(const double* part, int blocks, double* acc) {
-    if (threadIdx.x >= 4) return;
-    double s = 0.0;
-    for (int b = 0; b < blocks; ++b) s += part[4 * b + threadIdx.x];
-    acc[threadIdx.x] += s;
+    accumulate4(part, blocks, acc);
 }
 
 }  // namespace clipamd

@@ -706,8 +706,32 @@ struct SolverImpl : Impl {
     bool adj_ready = false;
 
     size_t state_elems() const { return (size_t)g.nodes * (2 * QL + 1 + L::D); }
+    // pool: kept states 0, K, 2K, ... | the final state T | segment states
     CT* snap_ptr(int k) { return adj_pool + (size_t)k * state_elems(); }
-    CT* seg_ptr(int j) { return adj_pool + (size_t)(adj_nsnap + j) * state_elems(); }
+    CT* final_ptr() { return adj_pool + (size_t)adj_nsnap * state_elems(); }
+    CT* seg_ptr(int j) { return adj_pool + (size_t)(adj_nsnap + 1 + j) * state_elems(); }
+    // parts of a kept state: f*, g*, phi (with its ghost ring), u
+    CT* st_f(CT* st) const { return st; }
+    CT* st_g(CT* st) const { return st + (size_t)g.nodes * QL; }
+    CT* st_phi(CT* st) const { return st + (size_t)g.nodes * 2 * QL; }
+    CT* st_u(CT* st) const { return st + (size_t)g.nodes * (2 * QL + 1); }
+
+    // One forward step from kept state `a` into kept state `b` (no copies).
+    void step_into(CT* a, CT* b) {
+        if constexpr (kPlain) {
+            StepArgs<ST, CT> s_ = args(cur, ucur, pcur, st_phi(b), st_phi(a));
+            s_.fin = (const ST*)st_f(a);
+            s_.gin = (const ST*)st_g(a);
+            s_.fout = (ST*)st_f(b);
+            s_.gout = (ST*)st_g(b);
+            s_.uin = st_u(a);
+            s_.ucol = st_u(a);
+            s_.uout = st_u(b);
+            ghost_refresh(st_phi(a), st_phi(b));
+            launch_phase_pull(s_);
+            launch_fused_pull<MODE_FULL>(s_);
+        }
+    }
     long long interior_count() const {
         long long t = 1;
         for (int a = 0; a < L::D; ++a) t *= g.dmax[a] - g.dmin[a] + 1;
@@ -790,7 +814,7 @@ struct SolverImpl : Impl {
                     h[g.idx(i, j, k)] = h[g.idx(w[0], w[1], w[2])];
                 }
         const int nsnap = (nsteps + every - 1) / every;
-        const size_t need = (size_t)(nsnap + every) * state_elems();
+        const size_t need = (size_t)(nsnap + 1 + every) * state_elems();
         if (need > adj_capacity) {
             adj_free();
             adj_pool = adj_alloc<CT>(need);
@@ -817,6 +841,7 @@ struct SolverImpl : Impl {
             step(1);
             if (n % every == 0 && n < nsteps) save_state(snap_ptr(n / every));
         }
+        save_state(final_ptr());
         std::vector<double> phi;
         download(phib[phc], (size_t)g.nodes, phi);
         out.assign((size_t)interior_count(), 0.0);
@@ -825,14 +850,16 @@ struct SolverImpl : Impl {
         step_count = nsteps;
     }
 
-    AdjArgs<CT> adj_args(const CT* phin) {
+    // Arguments of the reverse of the step from kept state `st` (phin: phi of
+    // the state after it, i.e. the step's stencil input with its lagged ring).
+    AdjArgs<CT> adj_args(CT* st, const CT* phin) {
         AdjArgs<CT> A;
         A.g = g;
         A.ph = ph;
-        A.F = fbuf[cur];
-        A.G = gbuf[cur];
+        A.F = st_f(st);
+        A.G = st_g(st);
         A.phin = phin;
-        A.uo = u[ucur];
+        A.uo = st_u(st);
         A.lF = lF;
         A.lG = lG;
         A.lU = lU;
@@ -848,21 +875,17 @@ struct SolverImpl : Impl {
         return A;
     }
 
-    // Reverse of the step that maps the loaded state S^n to S^{n+1}.
-    void adjoint_step() {
-        CT* phi_old = phib[phc];
-        CT* phi_new = phib[1 - phc];
-        ghost_refresh(phi_old, phi_new);
-        launch_phase_pull(args(cur, ucur, pcur, phi_new, phi_old));
-        const AdjArgs<CT> A = adj_args(phi_new);
+    // Reverse of the step from kept state `st` to the state whose phi is
+    // `phin`.  The partial sums of the parameter adjoints are folded into
+    // the accumulator by k_adj_pull's first block.
+    void adjoint_step(CT* st, const CT* phin) {
+        const AdjArgs<CT> A = adj_args(st, phin);
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
         k_adj_node<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_node");
-        k_adj_accumulate<CT><<<1, 64, 0, stream>>>(adj_part, (int)blocks, adj_acc);
-        launch_check("k_adj_accumulate");
         k_adj_phi<L, CT, true><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_phi");
-        k_adj_pull<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
+        k_adj_pull<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
         launch_check("k_adj_pull");
     }
 
@@ -892,23 +915,21 @@ struct SolverImpl : Impl {
             for_interior([&](long long t, long long n) { lp[n] = gout[t]; });
             upload(lPhi, lp);
         }
+        // segment k: states s0..s1-1 recomputed from the kept state s0 straight
+        // into the segment slots, then swept in reverse; the phi of state n+1
+        // is the stencil input of step n (its ghost ring is the lagged one)
         for (int k = adj_nsnap - 1; k >= 0; --k) {
             const int s0 = k * adj_K;
             const int s1 = std::min(adj_T, s0 + adj_K);
-            load_state(snap_ptr(k));
-            save_state(seg_ptr(0));
-            for (int j = 1; j < s1 - s0; ++j) {
-                step(1);
-                save_state(seg_ptr(j));
-            }
+            auto state = [&](int n) -> CT* { return n == s0 ? snap_ptr(k) : seg_ptr(n - s0); };
+            for (int n = s0 + 1; n < s1; ++n) step_into(state(n - 1), state(n));
             for (int n = s1 - 1; n >= s0; --n) {
-                load_state(seg_ptr(n - s0));
-                adjoint_step();
+                CT* after = (n + 1 < s1) ? state(n + 1) : (s1 < adj_T ? snap_ptr(k + 1) : final_ptr());
+                adjoint_step(state(n), st_phi(after));
             }
         }
-        // initial state and first collision
-        load_state(snap_ptr(0));
-        const AdjArgs<CT> A = adj_args(phib[phc]);
+        // initial state and first collision (phi0 with its periodic ring)
+        const AdjArgs<CT> A = adj_args(snap_ptr(0), st_phi(snap_ptr(0)));
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
         k_adj_init<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_init");
