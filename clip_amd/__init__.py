@@ -197,6 +197,10 @@ class Solver:
         _abi.check(self._lib.clip_check_health(self._h, ctypes.byref(out)))
         return out.value
 
+    def write_vts(self, path):
+        """VTSwriter: Velocity, C, P, Rho of the interior nodes as ASCII VTK."""
+        _abi.check(self._lib.clip_write_vts(self._h, str(path).encode()))
+
     def checkpoint_save(self, directory):
         _abi.check(self._lib.clip_checkpoint_save(self._h, str(directory).encode()))
 

@@ -1,8 +1,9 @@
 // clip_main.cpp — the `clip` executable with the reference driver's shape
 // (src/main.cu:39-91): read ./config.txt, fresh start or `-resume`, then the
-// step loop with reporting and checkpoints.  Differences: no VTS output (the
-// reference copies fields to the host every step for it, VTSwriter.cu:72-79),
-// and optional `-precision f64|f32`, `-config <path>`, `-steps <n>` flags.
+// step loop with VTS output, reporting and checkpoints.  Differences: fields
+// are copied to the host only on output steps (the reference copies them
+// every step, VTSwriter.cu:72-79); optional `-precision f64|f32|mixed_f16|
+// mixed_bf16`, `-config <path>`, `-steps <n>` and `-novts` flags.
 #include "clip_amd.hpp"
 
 #include <chrono>
@@ -15,11 +16,13 @@ int main(int argc, char** argv) {
     std::string config = "config.txt";
     int precision = CLIP_PREC_F64;
     long long steps_override = -1;
+    bool vts = true;
     for (int i = 1; i < argc; ++i) {
         const std::string a(argv[i]);
         if (a == "-resume") resume = true;
         else if (a == "-config" && i + 1 < argc) config = argv[++i];
         else if (a == "-steps" && i + 1 < argc) steps_override = std::atoll(argv[++i]);
+        else if (a == "-novts") vts = false;
         else if (a == "-precision" && i + 1 < argc) {
             const std::string p(argv[++i]);
             precision = (p == "f32") ? CLIP_PREC_F32 : (p == "mixed_f16") ? CLIP_PREC_MIXED_F16
@@ -35,12 +38,14 @@ int main(int argc, char** argv) {
         clip::TimeInfo timeInfo(inputData);
         clip::Reporter reporter(ns, inputData, timeInfo);
         clip::CheckPointer checkpointer(ns, inputData, timeInfo);
+        clip::VTSwriter vtsWriter(ns, inputData, timeInfo, "results", "results");
         if (resume) {
             checkpointer.load();
         } else {
             ns.initialCondition();
             ns.deviceInitializer();
         }
+        if (vts) vtsWriter.writeToFile();
         const long long final_step = steps_override >= 0 ? timeInfo.getCurrentStep() + steps_override
                                                          : (long long)timeInfo.getFinalStep();
         const auto t0 = std::chrono::steady_clock::now();
@@ -49,6 +54,7 @@ int main(int argc, char** argv) {
             ns.solve();
             timeInfo.increment();
             ++done;
+            if (vts) vtsWriter.writeToFile();
             reporter.print();
             checkpointer.save();
         }

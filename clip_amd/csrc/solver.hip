@@ -285,6 +285,66 @@ int clip_adjoint_set_normal_floor(clip_solver* s, double normal_floor) {
     return guard([&] { s->impl->adjoint_options(normal_floor); });
 }
 
+// VTSwriter::writeVTSBinaryFile (VTSwriter.cu:84-205): one ASCII VTK
+// StructuredGrid file over the interior nodes with Velocity, C, P and Rho.
+// The fields leave the device only here (the reference copies them to the
+// host every step, VTSwriter.cu:72-79).
+int clip_write_vts(clip_solver* s, const char* path) {
+    if (!s || !path) return set_error(CLIP_ERR_ARG, "null argument");
+    if (!s->impl->initialized) return set_error(CLIP_ERR_STATE, "solver not initialised");
+    return guard([&] {
+        const clip_config& c = s->cfg;
+        const int D = c.dim;
+        unsigned e[3];
+        extents(c, e);
+        std::vector<double> phi, p, rho, u;
+        s->impl->get_field(CLIP_FIELD_PHI, phi);
+        s->impl->get_field(CLIP_FIELD_P, p);
+        s->impl->get_field(CLIP_FIELD_RHO, rho);
+        s->impl->get_field(CLIP_FIELD_U, u);
+        const size_t nodes = (size_t)e[0] * e[1] * e[2];
+        unsigned lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = (a < D) ? 1 : 0;
+            hi[a] = (a < D) ? e[a] - 2 : 0;
+        }
+        std::ofstream f(path);
+        if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+        char ext[160];
+        std::snprintf(ext, sizeof(ext), "%u %u %u %u %u %u", lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+        f << "<?xml version=\"1.0\"?>\n";
+        f << "<VTKFile type=\"StructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+        f << "<StructuredGrid WholeExtent=\"" << ext << "\">\n";
+        f << "<Piece Extent=\"" << ext << "\">\n";
+        auto each = [&](const std::function<void(size_t)>& fn) {
+            for (unsigned k = lo[2]; k <= hi[2]; ++k)
+                for (unsigned j = lo[1]; j <= hi[1]; ++j)
+                    for (unsigned i = lo[0]; i <= hi[0]; ++i) fn(i + (size_t)e[0] * (j + (size_t)e[1] * k));
+        };
+        f << "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n";
+        for (unsigned k = lo[2]; k <= hi[2]; ++k)
+            for (unsigned j = lo[1]; j <= hi[1]; ++j)
+                for (unsigned i = lo[0]; i <= hi[0]; ++i) f << double(i) << " " << double(j) << " " << double(k) << "\n";
+        f << "</DataArray>\n</Points>\n";
+        f << "<PointData Scalars=\"scalars\" Vectors=\"velocity\">\n";
+        f << "<DataArray type=\"Float64\" Name=\"Velocity\" NumberOfComponents=\"" << D << "\" format=\"ascii\">\n";
+        each([&](size_t n) {
+            f << u[n] << " " << u[nodes + n];
+            if (D == 3) f << " " << u[2 * nodes + n];
+            f << "\n";
+        });
+        f << "</DataArray>\n";
+        const std::pair<const char*, const std::vector<double>*> scal[3] = {{"C", &phi}, {"P", &p}, {"Rho", &rho}};
+        for (const auto& sc : scal) {
+            f << "<DataArray type=\"Float64\" Name=\"" << sc.first << "\" format=\"ascii\">\n";
+            each([&](size_t n) { f << (*sc.second)[n] << "\n"; });
+            f << "</DataArray>\n";
+        }
+        f << "</PointData>\n</Piece>\n</StructuredGrid>\n</VTKFile>\n";
+        if (!f) throw std::runtime_error(std::string("write failed: ") + path);
+    });
+}
+
 void clip_destroy(clip_solver* s) { delete s; }
 
 }  // extern "C"

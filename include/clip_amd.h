@@ -201,6 +201,11 @@ int clip_get_info(const clip_solver* s, clip_info* out);
 int clip_last_step_timing(const clip_solver* s, double* total_ms, double* ghost_ms,
                           double* phase_ms, double* fused_ms);
 int clip_enable_timing(clip_solver* s, int on);
+/* VTSwriter::writeVTSBinaryFile (src/io/VTSwriter.cu:84-205): write the
+ * current Velocity, C (phi), P and Rho of the interior nodes to one ASCII VTK
+ * StructuredGrid file at `path`.  Fields leave the device only on this call. */
+int clip_write_vts(clip_solver* s, const char* path);
+
 /* ---- reverse mode (adjoint) ----------------------------------------------
  * No counterpart in the reference: the differentiable-solver extension for
  * gradient-based callers (parameter fitting, inverse problems).  Fully

@@ -18,6 +18,7 @@
 #include "clip_amd.h"
 
 #include <cstdio>
+#include <filesystem>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -158,6 +159,27 @@ private:
     NSAllen* m_ns;
     unsigned m_interval;
     const TimeInfo* m_ti;
+};
+
+// VTSwriter (VTSwriter.cu:20-205): results/results_t<step>.vts every
+// outputInterval steps; the fields leave the device only on those steps.
+class VTSwriter {
+public:
+    VTSwriter(NSAllen& ns, const InputData& idata, const TimeInfo& ti, const std::string& folder = "results",
+              const std::string& baseName = "results")
+        : m_ns(&ns), m_interval(idata.params.output_interval), m_ti(&ti), m_folder(folder), m_base(baseName) {}
+    void writeToFile() {
+        if (m_interval == 0 || m_ti->getCurrentStep() % m_interval != 0) return;
+        std::filesystem::create_directories(m_folder);
+        const std::string path = m_folder + "/" + m_base + "_t" + std::to_string(m_ti->getCurrentStep()) + ".vts";
+        check(clip_write_vts(m_ns->handle(), path.c_str()));
+    }
+
+private:
+    NSAllen* m_ns;
+    unsigned m_interval;
+    const TimeInfo* m_ti;
+    std::string m_folder, m_base;
 };
 
 class CheckPointer {

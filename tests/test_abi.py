@@ -57,16 +57,14 @@ def test_create_without_device_fails_loudly():
     import pytest
     from clip_amd import Config, Solver, ClipError
     from tests import cases
-    try:
-        import torch
-        if torch.cuda.is_available():
-            pytest.skip("a HIP device is present")
-    except ImportError:
-        pass
     cfg = Config.from_string(cases.drop2d(n=16, radius=4))
-    with pytest.raises(ClipError) as e:
-        Solver(cfg)
-    assert e.value.code in (_abi.ERR_HIP, _abi.ERR_NOMEM)
+    try:
+        s = Solver(cfg)
+    except ClipError as e:
+        assert e.code in (_abi.ERR_HIP, _abi.ERR_NOMEM)
+        return
+    s.close()
+    pytest.skip("a HIP device is present")
 
 
 def test_error_paths_without_device():

@@ -128,11 +128,41 @@ def test_driver_runs_config(tmp_path):
     exe = os.path.join(os.path.dirname(__file__), "..", "clip_amd", "clip")
     if not os.path.exists(exe):
         pytest.skip("driver not built")
-    (tmp_path / "config.txt").write_text(cases.drop2d(n=24, radius=6) + "checkpointInterval = 10\ncheckpointCopy = 2\n")
+    (tmp_path / "config.txt").write_text(cases.drop2d(n=24, radius=6)
+                                         + "checkpointInterval = 10\ncheckpointCopy = 2\noutputInterval = 10\n")
     r = subprocess.run([exe, "-steps", "20"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "MLUPS" in r.stdout
     assert os.path.exists(tmp_path / "Checkpoint" / "dev_f.bin")
     assert os.path.exists(tmp_path / "Checkpoint_1" / "dev_f.bin")
+    for t in (0, 10, 20):
+        assert os.path.exists(tmp_path / "results" / f"results_t{t}.vts")
     r = subprocess.run([exe, "-resume", "-steps", "5"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def test_vts_output_holds_the_fields(tmp_path):
+    """VTSwriter layout (VTSwriter.cu:84-205): StructuredGrid over the interior
+    extent, Velocity (DIM components), C, P, Rho, x fastest."""
+    import xml.etree.ElementTree as ET
+    for text in (cases.drop2d(n=12, radius=3), cases.bubble3d(n=8, radius=2.5)):
+        cfg = Config.from_string(text)
+        s = Solver(cfg, "f64").init_fresh().step(7)
+        path = tmp_path / "out.vts"
+        s.write_vts(path)
+        root = ET.parse(path).getroot()
+        grid = root.find("StructuredGrid")
+        ex = cfg.extents()
+        hi = [e - 2 for e in ex[: cfg.dim]] + [0] * (3 - cfg.dim)
+        lo = [1] * cfg.dim + [0] * (3 - cfg.dim)
+        assert grid.get("WholeExtent").split() == [str(v) for a in range(3) for v in (lo[a], hi[a])]
+        arrays = {d.get("Name"): np.array(d.text.split(), dtype=float) for d in grid.iter("DataArray") if d.get("Name")}
+        sl = cfg.interior_slices()
+
+        def flat(a):  # grid (i, j, k, dof) -> k, j, i order, dof fastest
+            return np.transpose(a[sl], (2, 1, 0, 3)).reshape(-1, a.shape[-1])
+
+        assert np.allclose(arrays["C"], flat(s.get("phi"))[:, 0], rtol=1e-5, atol=1e-12)
+        assert np.allclose(arrays["P"], flat(s.get("p"))[:, 0], rtol=1e-5, atol=1e-12)
+        assert np.allclose(arrays["Rho"], flat(s.get("rho"))[:, 0], rtol=1e-5, atol=1e-12)
+        assert np.allclose(arrays["Velocity"].reshape(-1, cfg.dim), flat(s.get("u")), rtol=1e-5, atol=1e-12)
