@@ -152,6 +152,9 @@ class Solver:
         _abi.check(self._lib.clip_synchronize(self._h))
 
     def set_stream(self, stream_ptr):
+        """Bind the handle to a hipStream_t of the HIP runtime this library
+        links (/opt/rocm).  A PyTorch wheel that bundles its own libamdhip64
+        has its own streams, which are not valid here."""
         _abi.check(self._lib.clip_set_stream(self._h, ctypes.c_void_p(int(stream_ptr))))
 
     def info(self):

@@ -39,6 +39,27 @@ def test_resume_is_bit_exact(tmp_path, text, precision):
         assert np.array_equal(got[sl], v[sl]), f
 
 
+@pytest.mark.parametrize("precision", ["mixed_f16", "mixed_bf16"])
+def test_mixed_resume_is_close(tmp_path, precision):
+    """Checkpoints hold FP64 values; re-encoding a 16-bit deviation from its
+    decoded FP32 value could in principle move a small stored value by one
+    unit in the last place, so only closeness is required (the resumed runs
+    of these cases match bit for bit)."""
+    cfg = Config.from_string(cases.drop2d(n=24, radius=6))
+    a = Solver(cfg, precision).init_fresh().step(30)
+    d = tmp_path / "Checkpoint"
+    a.checkpoint_save(d)
+    a.step(40)
+    b = Solver(cfg, precision)
+    b.checkpoint_load(d)
+    b.step(40)
+    for f in ("phi", "u"):
+        x, y = b.interior(f), a.interior(f)
+        err = np.linalg.norm(x - y) / np.linalg.norm(y)
+        print(precision, f, "resume rel-L2", err, flush=True)
+        assert err <= 1e-5, (f, err)
+
+
 def test_file_set_and_layout(tmp_path):
     cfg = Config.from_string(cases.jet2d(20, 24, radius=4))
     s = Solver(cfg, "f64").init_fresh().step(5)
