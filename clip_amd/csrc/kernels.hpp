@@ -689,8 +689,27 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) refq[q] = nbq[q];
         } else {
+            // ref_index with the wraps as per-axis deltas (computed once)
+            long long dp[3], dm[3];  // delta for e_a = +1 / -1
+            const long long st3[3] = {1, g.sy, g.sz};
 #pragma unroll
-            for (int q = 0; q < Q; ++q) refq[q] = a.phi[ref_index<L>(g, a.bc, c, q)];
+            for (int ax = 0; ax < 3; ++ax) {
+                const bool bp = ax < L::D && a.bc.both_periodic[ax];
+                const long long span = (long long)(g.dmax[ax] - g.dmin[ax] + 1) * st3[ax];
+                dp[ax] = (bp && c[ax] == g.dmax[ax]) ? -span : 0;
+                dm[ax] = (bp && c[ax] == g.dmin[ax]) ? span : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                long long r = n + L::ex(q) + L::ey(q) * g.sy + L::ez(q) * g.sz;
+#pragma unroll
+                for (int ax = 0; ax < L::D; ++ax) {
+                    const int e = L::e(q, ax);
+                    if (e > 0) r += dp[ax];
+                    if (e < 0) r += dm[ax];
+                }
+                refq[q] = a.phi[r];
+            }
         }
     }
     auto gout_ref = [&](int q) -> CT {
