@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -41,16 +42,20 @@ def run(cmd):
 def build(verbose=True):
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src, extra in HIP_SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        run([hipcc(), "-c", *COMMON, *DEVICE_FLAGS, *extra, os.path.join(CSRC, src), "-o", obj])
+        cmds.append([hipcc(), "-c", *COMMON, *DEVICE_FLAGS, *extra, os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CPP_SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        run(["g++", "-c", *COMMON, "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
-             "-I", "/opt/rocm/include", os.path.join(CSRC, src), "-o", obj])
+        cmds.append(["g++", "-c", *COMMON, "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+                     "-I", "/opt/rocm/include", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # translation units compile concurrently (each is one process)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(run, cmds))
     tmp = LIB + ".tmp"
     run([hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp])
     os.replace(tmp, LIB)

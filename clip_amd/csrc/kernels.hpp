@@ -29,6 +29,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "node.hpp"
 
 namespace clipamd {
@@ -571,27 +573,88 @@ __device__ __forceinline__ long long pull_src(long long n, const PullOffsets& o,
     return s;
 }
 
+// Bulk addressing of whole-row blocks (blockDim.y == 1): the row, the plane
+// and their periodic wraps are the same for every lane of a block, only x
+// varies.  Population addresses then split into a wave-uniform base (q plane,
+// row, y/z offset) and a 32-bit lane offset shared by all q with the same e_x.
+struct RowPull {
+    long long row;              // c1 sy + c2 sz
+    long long ym, yp, zm, zp;   // source offsets for e_y = +1 / -1, e_z = +1 / -1
+    unsigned x, xm, xp;         // source x for e_x = 0, +1, -1
+};
+template <class L>
+__device__ __forceinline__ RowPull row_pull(const DevGrid& g, const int c[3]) {
+    RowPull r;
+    r.row = c[1] * g.sy + c[2] * g.sz;
+    const long long spy = (long long)(g.dmax[1] - g.dmin[1]) * g.sy;
+    const long long spz = (long long)(g.dmax[2] - g.dmin[2]) * g.sz;
+    r.ym = (c[1] == g.dmin[1]) ? spy : -g.sy;
+    r.yp = (c[1] == g.dmax[1]) ? -spy : g.sy;
+    r.zm = (L::D == 3 && c[2] == g.dmin[2]) ? spz : -g.sz;
+    r.zp = (L::D == 3 && c[2] == g.dmax[2]) ? -spz : g.sz;
+    r.x = (unsigned)c[0];
+    r.xm = (unsigned)((c[0] == g.dmin[0]) ? g.dmax[0] : c[0] - 1);
+    r.xp = (unsigned)((c[0] == g.dmax[0]) ? g.dmin[0] : c[0] + 1);
+    return r;
+}
+template <class L>
+__device__ __forceinline__ long long row_uniform(const RowPull& r, int q) {
+    long long u = r.row;
+    if (L::ey(q) > 0) u += r.ym;
+    if (L::ey(q) < 0) u += r.yp;
+    if (L::ez(q) > 0) u += r.zm;
+    if (L::ez(q) < 0) u += r.zp;
+    return u;
+}
+template <class L>
+__device__ __forceinline__ unsigned row_lane(const RowPull& r, int q) {
+    return L::ex(q) > 0 ? r.xm : (L::ex(q) < 0 ? r.xp : r.x);
+}
+// Element at (uniform base) + (32-bit lane index).
+template <class T>
+__device__ __forceinline__ T* at_lane(T* base, unsigned idx) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return (T*)((B*)base + idx * (unsigned)sizeof(T));
+}
+
+template <class L>
+__device__ __forceinline__ bool node_of_thread_rows(const DevGrid& g, const DevBC& bc, int c[3]) {
+    const int ix = !bc.both_periodic[0], iy = !bc.both_periodic[1];
+    const int iz = (L::D == 3) ? !bc.both_periodic[2] : 0;
+    c[0] = blockIdx.x * blockDim.x + threadIdx.x + g.dmin[0] + ix;
+    c[1] = blockIdx.y + g.dmin[1] + iy;
+    c[2] = blockIdx.z + g.dmin[2] + iz;
+    return c[0] <= g.dmax[0] - ix;
+}
+
 // Node selection of the step kernels: SHELL = boundary layer (resolved pulls),
 // !SHELL && !DIRECT = bulk (offset pulls, periodic wrap), DIRECT = every
 // interior node (values read in place).
-template <class L, bool DIRECT, bool SHELL>
+template <class L, bool DIRECT, bool SHELL, bool ROWS = false>
 __device__ __forceinline__ bool select_node(const DevGrid& g, const DevBC& bc, const ShellDesc& sd, int c[3]) {
     if (SHELL) return shell_node<L>(g, sd, c);
+    if (ROWS) return node_of_thread_rows<L>(g, bc, c);
     return node_of_thread<L>(g, bc, DIRECT ? 0 : 1, c);
 }
 
 // ------------------------------------------------------------ step kernels
 // phi^{n+1} = sum_q g_post (kernelMacroscopicg, NsAllen.cu:516-540).
-template <class L, class ST, class CT, class SP, bool DIRECT, bool SHELL>
+template <class L, class ST, class CT, class SP, bool DIRECT, bool SHELL, bool ROWS = false>
 __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
     int c[3];
-    if (!select_node<L, DIRECT, SHELL>(a.g, a.bc, a.shell, c)) return;
+    if (!select_node<L, DIRECT, SHELL, ROWS>(a.g, a.bc, a.shell, c)) return;
     const long long n = a.g.idx(c[0], c[1], c[2]);
     CT sum = CT(0);
     if (DIRECT) {
 #pragma unroll
         for (int q = 0; q < L::Q; ++q)
             sum += SP::ld(a.gin, (long long)q * a.g.nodes + n, q, gref<SP>(a.phi_ref, ref_index<L>(a.g, a.bc, c, q)));
+    } else if (!SHELL && ROWS) {
+        const RowPull rp = row_pull<L>(a.g, c);
+        const CT r = gref<SP>(a.phi_ref, n);  // every pulled g is referenced to this node
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q)
+            sum += SP::ld(at_lane(a.gin + (long long)q * a.g.nodes + row_uniform<L>(rp, q), row_lane<L>(rp, q)), 0, q, r);
     } else if (!SHELL) {
         const PullOffsets po = pull_offsets<L>(a.g, c);
         const CT r = gref<SP>(a.phi_ref, n);  // every pulled g is referenced to this node
@@ -659,12 +722,12 @@ enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 // normal_FD 241-275, kernelCollisionMRTg 545-584, kernelCollisionMRTf 589-763).
 // FP64: the reference's transforms and operation order.  FP32 / mixed: the
 // exact moment-space form of the same step (node.hpp).
-template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL>
+template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS = false>
 __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     constexpr int Q = L::Q;
     constexpr bool MOMENT_FORM = sizeof(CT) == 4;
     int c[3];
-    if (!select_node<L, DIRECT, SHELL>(a.g, a.bc, a.shell, c)) return;
+    if (!select_node<L, DIRECT, SHELL, ROWS>(a.g, a.bc, a.shell, c)) return;
     const long long shell_t = SHELL ? (long long)blockIdx.x * blockDim.x + threadIdx.x : 0;
     const DevGrid& g = a.g;
     const Phys<CT>& ph = a.ph;
@@ -718,7 +781,18 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     };
 
     PullOffsets po;
-    if (!DIRECT && !SHELL) po = pull_offsets<L>(g, c);
+    if (!DIRECT && !SHELL && !ROWS) po = pull_offsets<L>(g, c);
+    RowPull rp;
+    if (ROWS) rp = row_pull<L>(g, c);
+    // population q of this node (stores) and of its pull source (bulk loads)
+    auto dst_ptr = [&](ST* A, int q) -> ST* {
+        if constexpr (ROWS) return at_lane(A + (long long)q * g.nodes + rp.row, rp.x);
+        else return A + (long long)q * g.nodes + n;
+    };
+    auto src_ptr = [&](const ST* A, int q) -> const ST* {
+        if constexpr (ROWS) return at_lane(A + (long long)q * g.nodes + row_uniform<L>(rp, q), row_lane<L>(rp, q));
+        else return A + (long long)q * g.nodes + pull_src<L>(n, po, q);
+    };
 
     // f_post
     CT fp[Q];
@@ -727,7 +801,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         for (int q = 0; q < Q; ++q) {
             if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q, CT(0));
             else if (!SHELL)
-                fp[q] = SP::ld(a.fin, (long long)q * g.nodes + pull_src<L>(n, po, q), q, CT(0));
+                fp[q] = SP::ld(src_ptr(a.fin, q), 0, q, CT(0));
             else
                 fp[q] = a.shell_f[(long long)q * shell_total(a.shell) + shell_t];
         }
@@ -740,10 +814,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     const CT rn = (!DIRECT && !SHELL) ? gref<SP>(a.phi_ref, n) : CT(0);
     auto pull_g = [&](int q) -> CT {
         if (DIRECT) return SP::ld(a.gin, (long long)q * g.nodes + n, q, gref<SP>(a.phi_ref, ref_index<L>(g, a.bc, c, q)));
-        if (!SHELL) {
-            const long long src = pull_src<L>(n, po, q);
-            return SP::ld(a.gin, (long long)q * g.nodes + src, q, rn);
-        }
+        if (!SHELL) return SP::ld(src_ptr(a.gin, q), 0, q, rn);
         return a.shell_g[(long long)q * shell_total(a.shell) + shell_t];
     };
 
@@ -780,8 +851,8 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         }
         collide_node<L, CT>(
             z, p, Pf, c0, dc, u, gp,
-            [&](int q, CT v) { SP::st(a.gout, (long long)q * g.nodes + n, q, v, gout_ref(q)); },
-            [&](int q, CT v) { SP::st(a.fout, (long long)q * g.nodes + n, q, v, CT(0)); });
+            [&](int q, CT v) { SP::st(dst_ptr(a.gout, q), 0, q, v, gout_ref(q)); },
+            [&](int q, CT v) { SP::st(dst_ptr(a.fout, q), 0, q, v, CT(0)); });
         return;
     } else {
         const CT mu = CT(4) * ph.beta * c0 * (c0 - CT(1)) * (c0 - CT(0.5)) - ph.kappa * lap;
@@ -851,7 +922,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
                 CT en = CT(L::ex(q)) * nr[0] + CT(L::ey(q)) * nr[1];
                 if (L::D == 3) en += CT(L::ez(q)) * nr[2];
                 const CT heq = c0 * (gamma_u<L, CT>(q, u) + CT(L::w(q))) + CT(L::w(q)) * (aterm * en);
-                SP::st(a.gout, (long long)q * g.nodes + n, q, gp * (CT(1) - wc) + heq * wc, CT(0));
+                SP::st(dst_ptr(a.gout, q), 0, q, gp * (CT(1) - wc) + heq * wc, CT(0));
             }
         }
 
@@ -878,7 +949,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
         }
         wmrt_filter<L, CT>(gneq, s);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) SP::st(a.fout, (long long)q * g.nodes + n, q, fp[q] - gneq[q] + hlp[q], CT(0));
+        for (int q = 0; q < Q; ++q) SP::st(dst_ptr(a.fout, q), 0, q, fp[q] - gneq[q] + hlp[q], CT(0));
     }
 }
 

@@ -271,7 +271,8 @@ struct SolverImpl : Impl {
     template <int MODE>
     void launch_fused_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
-        if (gb.x) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a);
+        if (gb.x && tby == 1) k_fused<L, ST, CT, SP, false, MODE, false, true><<<gb, block_shape(), 0, stream>>>(a);
+        else if (gb.x) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a);
         if (shell_blocks()) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
         launch_check("k_fused");
     }
@@ -279,7 +280,8 @@ struct SolverImpl : Impl {
     // which also leaves the layer's f_post / g_post for k_fused).
     void launch_phase_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
-        if (gb.x) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
+        if (gb.x && tby == 1) k_phase<L, ST, CT, SP, false, false, true><<<gb, block_shape(), 0, stream>>>(a);
+        else if (gb.x) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
         launch_check("k_phase");
         if (shell_blocks()) {
             launch_shell_pull(a);

@@ -128,6 +128,25 @@ def test_box3d_walls():
     assert_fields(gpu, orc)
 
 
+# Rows of 129..256 interior nodes run the bulk kernels with one block per row
+# (row, plane and their periodic wraps uniform per block); these cases keep
+# that addressing under the same bar as the rest.
+@pytest.mark.parametrize("name,text,steps", [
+    ("drop2d_200x40", cases.drop2d(n=200, ny=40, radius=12), 150),
+    ("box2d_160x32", cases.box2d([("x-", "wall"), ("x+", "do nothing"), ("y-", "free convect"),
+                                  ("y+", "periodic")], n=160, ny=32), 120),
+    ("box2d_140x24", cases.box2d([("x-", "periodic"), ("x+", "periodic"), ("y-", "wall"),
+                                  ("y+", "neumann")], n=140, ny=24), 120),
+    ("jet3d_140x12x10", cases.jet3d((140, 12, 10), radius=3), 60),
+    ("jet3d_c5_136x10x20", cases.jet3d_c5(nx=136, ny=10, nz=20, radius=4.0, L=8.0, U=0.02), 60),
+], ids=["drop2d_200x40", "box2d_160x32", "box2d_140x24", "jet3d_140x12x10", "jet3d_c5_136x10x20"])
+def test_whole_row_blocks(name, text, steps):
+    cfg, gpu, orc = run_pair(text, steps)
+    assert_fields(gpu, orc)
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL, f
+
+
 def test_sum_matches_interior_sum():
     cfg, gpu, orc = run_pair(cases.bubble3d(n=12, radius=3), 10)
     phi = gpu.interior("phi")
@@ -165,7 +184,8 @@ def test_fp32_against_fp64_oracle(name, text, steps, tol):
     ("drop2d", cases.drop2d(n=32, radius=8), 100),
     ("bubble3d", cases.bubble3d(n=16, radius=4), 60),
     ("jet2d", cases.jet2d(24, 32, radius=4), 100),
-], ids=["drop2d", "bubble3d", "jet2d"])
+    ("drop2d_200x40", cases.drop2d(n=200, ny=40, radius=12), 100),
+], ids=["drop2d", "bubble3d", "jet2d", "drop2d_200x40"])
 def test_mixed_against_fp64_oracle(precision, tol_phi, tol_u, name, text, steps):
     cfg, gpu, orc = run_pair(text, steps, precision=precision)
     a_phi, b_phi = gpu.interior("phi"), orc.interior("phi")
