@@ -184,14 +184,25 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a) {
     if (LAG) a.lPhi[n] = lag;
 }
 
-// Transpose of the pull: adjoint of f*, g* of S^n at interior node y.
-// acc[k] += sum over blocks of part[4 b + k], in block order.
+// acc[k] += sum over blocks of part[4 b + k], run by a whole block
+// (blockDim.x a power of two, 8..256; every thread must call it).  Thread t
+// sums k = t % 4 over b = t / 4, t / 4 + blockDim.x / 4, ... and the per-thread
+// sums meet in a fixed tree, so the result does not depend on timing.
 __device__ __forceinline__ void accumulate4(const double* part, int blocks, double* acc) {
-    if (threadIdx.x >= 4) return;
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
     double s = 0.0;
-    for (int b = 0; b < blocks; ++b) s += part[4 * b + threadIdx.x];
-    acc[threadIdx.x] += s;
+    for (int b = t >> 2; b < blocks; b += (int)(blockDim.x >> 2)) s += part[4 * b + (t & 3)];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = (int)blockDim.x >> 1; w >= 4; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];  // w is a multiple of 4: t and t + w share k
+        __syncthreads();
+    }
+    if (t < 4) acc[t] += sh[t];
 }
+
+// Transpose of the pull: adjoint of f*, g* of S^n at interior node y.
 
 template <class L, class CT>
 __global__ __launch_bounds__(ADJ_TB) void k_adj_pull(AdjArgs<CT> a, double* acc, int blocks) {
@@ -245,9 +256,9 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_init(AdjArgs<CT> a) {
     block_sum4(pb, pk, pm, ps, a.part);
 }
 
-// acc[k] += sum over blocks of part[4 b + k], in block order.
+// acc[k] += sum over blocks of part[4 b + k] (one block of ADJ_TB threads).
 template <class CT>
-__global__ void k_adj_accumulate(const double* part, int blocks, double* acc) {
+__global__ __launch_bounds__(ADJ_TB) void k_adj_accumulate(const double* part, int blocks, double* acc) {
     accumulate4(part, blocks, acc);
 }
 

@@ -124,7 +124,8 @@ struct ShellDesc {
 };
 __host__ __device__ inline long long shell_total(const ShellDesc& sd) {
     long long t = 0;
-    for (int k = 0; k < sd.na; ++k) t += sd.count[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t += (k < sd.na) ? sd.count[k] : 0;  // constant indices
     return t;
 }
 
@@ -511,31 +512,42 @@ __host__ __device__ inline ShellDesc make_shell(const DevGrid& g, const DevBC& b
 
 template <class L>
 __device__ __forceinline__ bool shell_node_at(const DevGrid& g, const ShellDesc& sd, long long t64, int c[3]) {
-    // 32-bit index arithmetic: a boundary layer holds far fewer than 2^32 nodes
+    // 32-bit index arithmetic: a boundary layer holds far fewer than 2^32
+    // nodes.  Every loop has constant bounds and every array index is a
+    // constant, so ShellDesc and c stay in registers.
     unsigned t = (unsigned)t64;
-    for (int k = 0; k < sd.na; ++k) {
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (found || k >= sd.na) continue;
         const unsigned cnt = (unsigned)sd.count[k];
         if (t >= cnt) {
             t -= cnt;
             continue;
         }
+        found = true;
         const int a = sd.axes[k];
         const unsigned half = cnt / 2;
         const bool hi = t >= half;
         unsigned r = hi ? t - half : t;
+#pragma unroll
         for (int b = 0; b < 3; ++b) {
             if (b == a) continue;
             bool earlier = false;
+#pragma unroll
             for (int m = 0; m < k; ++m) earlier |= sd.axes[m] == b;
             const int lo = g.dmin[b] + (earlier ? 1 : 0);
             const unsigned size = (unsigned)(g.dmax[b] - g.dmin[b] + 1 - (earlier ? 2 : 0));
             c[b] = lo + (int)(r % size);
             r /= size;
         }
-        c[a] = hi ? g.dmax[a] : g.dmin[a];
-        return true;
+        // arithmetic rather than a choice between two fields: a selected
+        // field address would force the kernel arguments into scratch
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (b == a) c[b] = g.dmin[b] + (hi ? g.dmax[b] - g.dmin[b] : 0);
     }
-    return false;
+    return found;
 }
 
 template <class L>
@@ -676,30 +688,56 @@ __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
 // scratch that the shell launch of k_fused reads, and the outflow planes of
 // the next step.  One thread per pair spreads the fix-up chains over the
 // chip; keeping them out of k_fused keeps that kernel within its registers.
+// blockIdx.y is q, so every wave resolves one compile-time direction.
+template <class L, class ST, class CT, class SP, int Q0>
+__device__ __forceinline__ void shell_pull_dir(const ST* fin, const ST* gin, const DevGrid& g, const DevBC& bc,
+                                               const CT* uin, const CT* const* prev_f_in,
+                                               const CT* const* prev_g_in, const CT* phi_ref, CT* shell_f,
+                                               CT* shell_g, CT* const* prev_f_out, CT* const* prev_g_out,
+                                               unsigned t, unsigned S, const int c[3]) {
+    Puller<L, ST, CT, SP, false> Pf{fin, g, bc, uin, prev_f_in, phi_ref};
+    Puller<L, ST, CT, SP, true> Pg{gin, g, bc, uin, prev_g_in, phi_ref};
+    const CT fv = Pf.lvl3(c, Q0);
+    const CT gv = Pg.lvl3(c, Q0);
+    shell_f[(long long)Q0 * S + t] = fv;
+    shell_g[(long long)Q0 * S + t] = gv;
+    if (bc.is_fc) {
+        store_prev<L, CT>(g, bc, prev_f_out, c, Q0, fv);
+        store_prev<L, CT>(g, bc, prev_g_out, c, Q0, gv);
+    }
+}
 template <class L, class ST, class CT, class SP>
 __global__ __launch_bounds__(BX* BY) void k_shell_pull(StepArgs<ST, CT> a) {
     const unsigned S = (unsigned)shell_total(a.shell);
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S * (unsigned)L::Q) return;
-    const unsigned t = i % S;
-    const int myq = (int)(i / S);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     int c[3];
-    if (!shell_node_at<L>(a.g, a.shell, t, c)) return;
-    const DevGrid& g = a.g;
-    Puller<L, ST, CT, SP, false> Pf{a.fin, g, a.bc, a.uin, a.prev_f_in, a.phi_ref};
-    Puller<L, ST, CT, SP, true> Pg{a.gin, g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
+    if (t >= S || !shell_node_at<L>(a.g, a.shell, t, c)) return;
+    // outflow-plane pointers as local arrays (constant indices keep them in registers)
+    const CT* pfi[6];
+    const CT* pgi[6];
+    CT* pfo[6];
+    CT* pgo[6];
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-        if (q != myq) continue;
-        const CT fv = Pf.lvl3(c, q);
-        const CT gv = Pg.lvl3(c, q);
-        a.shell_f[(long long)q * S + t] = fv;
-        a.shell_g[(long long)q * S + t] = gv;
-        if (a.bc.is_fc) {
-            store_prev<L, CT>(g, a.bc, a.prev_f_out, c, q, fv);
-            store_prev<L, CT>(g, a.bc, a.prev_g_out, c, q, gv);
-        }
+    for (int s = 0; s < 6; ++s) {
+        pfi[s] = a.prev_f_in[s];
+        pgi[s] = a.prev_g_in[s];
+        pfo[s] = a.prev_f_out[s];
+        pgo[s] = a.prev_g_out[s];
     }
+#define CLIP_SHELL_DIR(k)                                                                                   \
+    case k:                                                                                                 \
+        if constexpr (k < L::Q)                                                                             \
+            shell_pull_dir<L, ST, CT, SP, k>(a.fin, a.gin, a.g, a.bc, a.uin, pfi, pgi, a.phi_ref, a.shell_f, \
+                                             a.shell_g, pfo, pgo, t, S, c);                                 \
+        break;
+    switch (blockIdx.y) {
+        CLIP_SHELL_DIR(0) CLIP_SHELL_DIR(1) CLIP_SHELL_DIR(2) CLIP_SHELL_DIR(3) CLIP_SHELL_DIR(4)
+        CLIP_SHELL_DIR(5) CLIP_SHELL_DIR(6) CLIP_SHELL_DIR(7) CLIP_SHELL_DIR(8) CLIP_SHELL_DIR(9)
+        CLIP_SHELL_DIR(10) CLIP_SHELL_DIR(11) CLIP_SHELL_DIR(12) CLIP_SHELL_DIR(13) CLIP_SHELL_DIR(14)
+        CLIP_SHELL_DIR(15) CLIP_SHELL_DIR(16) CLIP_SHELL_DIR(17) CLIP_SHELL_DIR(18)
+        default: break;
+    }
+#undef CLIP_SHELL_DIR
 }
 
 // phi^{n+1} = sum_q g_post at the boundary-layer nodes (q ascending, as

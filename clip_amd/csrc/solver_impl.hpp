@@ -290,8 +290,7 @@ struct SolverImpl : Impl {
         }
     }
     void launch_shell_pull(const StepArgs<ST, CT>& a) {
-        const unsigned blocks = (unsigned)((shell_nodes() * QL + BX * BY - 1) / (BX * BY));
-        k_shell_pull<L, ST, CT, SP><<<blocks, BX * BY, 0, stream>>>(a);
+        k_shell_pull<L, ST, CT, SP><<<dim3(shell_blocks(), QL), BX * BY, 0, stream>>>(a);
         launch_check("k_shell_pull");
     }
 
@@ -935,7 +934,7 @@ struct SolverImpl : Impl {
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
         k_adj_init<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_init");
-        k_adj_accumulate<CT><<<1, 64, 0, stream>>>(adj_part, (int)blocks, adj_acc);
+        k_adj_accumulate<CT><<<1, ADJ_TB, 0, stream>>>(adj_part, (int)blocks, adj_acc);
         launch_check("k_adj_accumulate");
         k_adj_phi<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_phi");
