@@ -202,6 +202,8 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run an elementwise FP32 copy (read + write) of this many MiB five times; "
                          "its rocprofv3 FETCH_SIZE / WRITE_SIZE calibrate the traffic counters")
+    ap.add_argument("--adjoint-only", choices=["f32", "f64"], default=None,
+                    help="only run the reverse-mode measurement at this precision (for kernel traces)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -222,6 +224,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}),
               flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_measure(args.adjoint_only)), flush=True)
         return
     text = workload_text(args.workload)
     cfg = Config.from_string(text)
