@@ -754,6 +754,46 @@ __global__ __launch_bounds__(BX* BY) void k_shell_phi(StepArgs<ST, CT> a) {
     a.phi[a.g.idx(c[0], c[1], c[2])] = sum;
 }
 
+// Reference phase field of each stored g_q of node c (mixed storage): phi at
+// the node that pulls it next, c + e_q, wrapped along the axes periodic on
+// both ends.  nbq (phi at n + e_q, the stencil's own loads) is that value
+// away from those seams; at a seam the wraps are per-axis deltas.
+template <class L, class ST, class CT>
+__device__ __forceinline__ void g_store_refs(const StepArgs<ST, CT>& a, const int c[3], long long n, const CT nbq[],
+                                             CT refq[]) {
+    constexpr int Q = L::Q;
+    const DevGrid& g = a.g;
+    bool seam = false;
+#pragma unroll
+    for (int ax = 0; ax < L::D; ++ax)
+        seam |= a.bc.both_periodic[ax] && (c[ax] == g.dmin[ax] || c[ax] == g.dmax[ax]);
+    if (!seam) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) refq[q] = nbq[q];
+        return;
+    }
+    long long dp[3], dm[3];  // delta for e_a = +1 / -1
+    const long long st3[3] = {1, g.sy, g.sz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const bool bp = ax < L::D && a.bc.both_periodic[ax];
+        const long long span = (long long)(g.dmax[ax] - g.dmin[ax] + 1) * st3[ax];
+        dp[ax] = (bp && c[ax] == g.dmax[ax]) ? -span : 0;
+        dm[ax] = (bp && c[ax] == g.dmin[ax]) ? span : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        long long r = n + L::ex(q) + L::ey(q) * g.sy + L::ez(q) * g.sz;
+#pragma unroll
+        for (int ax = 0; ax < L::D; ++ax) {
+            const int e = L::e(q, ax);
+            if (e > 0) r += dp[ax];
+            if (e < 0) r += dm[ax];
+        }
+        refq[q] = a.phi[r];
+    }
+}
+
 enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 
 // Fused macroscopic + collision (kernelMacroscopicf NsAllen.cu:369-511,
@@ -778,41 +818,8 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
     CT nbq[Q];  // phi at n + e_q (the stencil's own loads)
     phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap, SP::uses_ref ? nbq : nullptr);
     // reference of the stored g (mixed policy): phi of the node that pulls it
-    bool seam = false;  // next to a side of an axis periodic on both ends
-    if constexpr (SP::uses_ref) {
-#pragma unroll
-        for (int ax = 0; ax < L::D; ++ax)
-            seam |= a.bc.both_periodic[ax] && (c[ax] == g.dmin[ax] || c[ax] == g.dmax[ax]);
-    }
     CT refq[SP::uses_ref ? Q : 1];
-    if constexpr (SP::uses_ref) {
-        if (!seam) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) refq[q] = nbq[q];
-        } else {
-            // ref_index with the wraps as per-axis deltas (computed once)
-            long long dp[3], dm[3];  // delta for e_a = +1 / -1
-            const long long st3[3] = {1, g.sy, g.sz};
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const bool bp = ax < L::D && a.bc.both_periodic[ax];
-                const long long span = (long long)(g.dmax[ax] - g.dmin[ax] + 1) * st3[ax];
-                dp[ax] = (bp && c[ax] == g.dmax[ax]) ? -span : 0;
-                dm[ax] = (bp && c[ax] == g.dmin[ax]) ? span : 0;
-            }
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                long long r = n + L::ex(q) + L::ey(q) * g.sy + L::ez(q) * g.sz;
-#pragma unroll
-                for (int ax = 0; ax < L::D; ++ax) {
-                    const int e = L::e(q, ax);
-                    if (e > 0) r += dp[ax];
-                    if (e < 0) r += dm[ax];
-                }
-                refq[q] = a.phi[r];
-            }
-        }
-    }
+    if constexpr (SP::uses_ref) g_store_refs<L>(a, c, n, nbq, refq);
     auto gout_ref = [&](int q) -> CT {
         if constexpr (!SP::uses_ref) return CT(0);
         else return refq[q];
@@ -989,6 +996,128 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) SP::st(dst_ptr(a.fout, q), 0, q, fp[q] - gneq[q] + hlp[q], CT(0));
     }
+}
+
+// ------------------------------------------------------------ node pairs
+// 16-bit storage moves one population of one node per lane and access, and
+// a 19-plane pull at that width is bound by the access count rather than by
+// bytes.  k_phase_pair gives each lane two x-adjacent bulk nodes, so each
+// pulled g of both is one 32-bit access (2-byte aligned in general; the pair
+// is split where a periodic wrap separates its sources).  Per node it sums
+// exactly as k_phase does.
+struct PairGrid {
+    int xlo, xhi;      // bulk x range
+    int ylo, ny, zlo;  // bulk rows: y = ylo + r % ny, z = zlo + r / ny
+    unsigned per_row;  // pairs per row: (xhi - xlo + 2) / 2
+    unsigned total;    // pairs in the bulk
+};
+
+// Pair number `pair`: node 0 at c, node 1 at c + (1,0,0) (absent when the
+// row has an odd length and this is its last pair).
+__device__ __forceinline__ void pair_coords(const PairGrid& pg, unsigned pair, int c[3], bool& has1) {
+    const unsigned r = pair / pg.per_row;
+    const unsigned p = pair - r * pg.per_row;
+    c[0] = pg.xlo + 2 * (int)p;
+    c[1] = pg.ylo + (int)(r % (unsigned)pg.ny);
+    c[2] = pg.zlo + (int)(r / (unsigned)pg.ny);
+    has1 = c[0] + 1 <= pg.xhi;
+}
+__device__ __forceinline__ bool pair_of_thread(const PairGrid& pg, int c[3], bool& has1) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pg.total) return false;
+    pair_coords(pg, t, c, has1);
+    return true;
+}
+
+// Pull sources of the pair along x for direction q (element index within the
+// row): node 0's source, wrapped on x when x is periodic on both ends, and
+// node 1's (node 0's + 1 when node 1 is absent, which stays inside the row).
+template <class L>
+__device__ __forceinline__ void pair_sources(const DevGrid& g, const int c[3], bool has1, int q, int& s0, int& s1) {
+    const int ex = L::ex(q);
+    s0 = c[0] - ex;
+    s1 = c[0] + 1 - ex;
+    // the bulk reaches dmin / dmax only along an x periodic on both ends
+    if (ex > 0 && c[0] == g.dmin[0]) s0 = g.dmax[0];
+    if (ex < 0 && c[0] == g.dmax[0]) s0 = g.dmin[0];  // node 0 last of an odd row
+    if (ex < 0 && c[0] + 1 == g.dmax[0]) s1 = g.dmin[0];
+    if (!has1) s1 = s0 + 1;
+}
+
+// Two 16-bit values at row + s0 (low half) and row + s1 (high half) of one plane.
+template <class ST>
+__device__ __forceinline__ unsigned load_pair_word(const ST* row, int s0, int s1) {
+    unsigned w;
+    if (s1 == s0 + 1) {
+        __builtin_memcpy(&w, row + s0, 4);
+    } else {
+        unsigned short h0, h1;
+        __builtin_memcpy(&h0, row + s0, 2);
+        __builtin_memcpy(&h1, row + s1, 2);
+        w = (unsigned)h0 | ((unsigned)h1 << 16);
+    }
+    return w;
+}
+template <class ST>
+__device__ __forceinline__ ST half_of(unsigned w, int hi) {
+    const unsigned short h = (unsigned short)(hi ? w >> 16 : w & 0xffffu);
+    ST v;
+    __builtin_memcpy(&v, &h, 2);
+    return v;
+}
+template <class ST>
+__device__ __forceinline__ void load_pair(const ST* row, int s0, int s1, ST& v0, ST& v1) {
+    const unsigned w = load_pair_word(row, s0, s1);
+    v0 = half_of<ST>(w, 0);
+    v1 = half_of<ST>(w, 1);
+}
+
+template <class ST>
+__device__ __forceinline__ void store_pair(ST* at, bool has1, ST v0, ST v1) {
+    if (has1) {
+        unsigned short lo, hi;
+        __builtin_memcpy(&lo, &v0, 2);
+        __builtin_memcpy(&hi, &v1, 2);
+        const unsigned w = (unsigned)lo | ((unsigned)hi << 16);
+        __builtin_memcpy(at, &w, 4);
+    } else {
+        at[0] = v0;
+    }
+}
+
+// Source row (y, z part of the pull) of direction q, relative to the pair's row.
+template <class L>
+__device__ __forceinline__ long long pair_row_offset(const PullOffsets& po, int q) {
+    long long s = 0;
+    if (L::ey(q) > 0) s += po.m[1];
+    if (L::ey(q) < 0) s += po.p[1];
+    if (L::ez(q) > 0) s += po.m[2];
+    if (L::ez(q) < 0) s += po.p[2];
+    return s;
+}
+
+template <class L, class ST, class SP>
+__global__ __launch_bounds__(BX* BY) void k_phase_pair(StepArgs<ST, float> a, PairGrid pg) {
+    int c[3];
+    bool has1;
+    if (!pair_of_thread(pg, c, has1)) return;
+    const DevGrid& g = a.g;
+    const long long n0 = g.idx(c[0], c[1], c[2]);
+    const long long row = n0 - c[0];
+    const PullOffsets po = pull_offsets<L>(g, c);
+    const float r0 = gref<SP>(a.phi_ref, n0), r1 = has1 ? gref<SP>(a.phi_ref, n0 + 1) : 0.0f;
+    float sum0 = 0.0f, sum1 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        int s0, s1;
+        pair_sources<L>(g, c, has1, q, s0, s1);
+        ST v0, v1;
+        load_pair(a.gin + (long long)q * g.nodes + row + pair_row_offset<L>(po, q), s0, s1, v0, v1);
+        sum0 += SP::ld(&v0, 0, q, r0);
+        sum1 += SP::ld(&v1, 0, q, r1);
+    }
+    a.phi[n0] = sum0;
+    if (has1) a.phi[n0 + 1] = sum1;
 }
 
 // ------------------------------------------------------------ phi ghost ring

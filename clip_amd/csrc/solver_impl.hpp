@@ -34,6 +34,13 @@ struct SolverImpl : Impl {
     // 2.566 ms; FP32 1.307 -> 1.367 ms and the 16-bit storage unchanged, so
     // only the FP64 instantiation uses it.
     static constexpr bool kRowAddressing = sizeof(CT) == 8;
+    // 16-bit storage: the bulk phase sum on x-adjacent node pairs
+    // (k_phase_pair), one 32-bit access per pulled g of both nodes.  Measured
+    // on the 256^3 bubble: 0.190 -> 0.164 ms.  The same pairing of k_fused
+    // (two nodes per lane: 192 VGPRs; or the f pair and the g pair on
+    // adjacent lanes with a lane swap) was slower than one node per lane
+    // (1.39 and 1.05 ms against 0.83 ms), so k_fused is not paired.
+    static constexpr bool kPairs = sizeof(ST) == 2;
     clip_config cfg;
     DevGrid g;
     DevBC bc;
@@ -258,6 +265,28 @@ struct SolverImpl : Impl {
         return dim3((n[0] + tbx - 1) / tbx, (n[1] + tby - 1) / tby, n[2]);
     }
     dim3 grid_interior() const { return grid_box(0); }
+    // The bulk box of grid_box(1) as rows of x-adjacent node pairs.
+    PairGrid pair_grid() const {
+        PairGrid pg{};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            const int in = (a < L::D && !bc.both_periodic[a]) ? 1 : 0;
+            lo[a] = g.dmin[a] + in;
+            hi[a] = g.dmax[a] - in;
+        }
+        if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return pg;
+        pg.xlo = lo[0];
+        pg.xhi = hi[0];
+        pg.ylo = lo[1];
+        pg.ny = hi[1] - lo[1] + 1;
+        pg.zlo = lo[2];
+        pg.per_row = (unsigned)((hi[0] - lo[0] + 2) / 2);
+        const unsigned long long total = (unsigned long long)pg.per_row * pg.ny * (hi[2] - lo[2] + 1);
+        if (total > 0x7fffffffull) throw ConfigError("bulk too large for the pair launch (2^31 pairs)");
+        pg.total = (unsigned)total;
+        return pg;
+    }
+    unsigned pair_blocks(const PairGrid& pg) const { return (pg.total + BX * BY - 1) / (BX * BY); }
     dim3 block_shape() const { return dim3(tbx, tby); }
     long long shell_nodes() const {
         const ShellDesc sd = make_shell<L>(g, bc);
@@ -276,12 +305,12 @@ struct SolverImpl : Impl {
     template <int MODE>
     void launch_fused_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
-        bool rows = false;
+        bool done = false;
         if constexpr (kRowAddressing) {
-            rows = gb.x && tby == 1;
-            if (rows) k_fused<L, ST, CT, SP, false, MODE, false, true><<<gb, block_shape(), 0, stream>>>(a);
+            done = gb.x && tby == 1;
+            if (done) k_fused<L, ST, CT, SP, false, MODE, false, true><<<gb, block_shape(), 0, stream>>>(a);
         }
-        if (gb.x && !rows) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a);
+        if (gb.x && !done) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a);
         if (shell_blocks()) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
         launch_check("k_fused");
     }
@@ -289,12 +318,17 @@ struct SolverImpl : Impl {
     // which also leaves the layer's f_post / g_post for k_fused).
     void launch_phase_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
-        bool rows = false;
+        bool done = false;
         if constexpr (kRowAddressing) {
-            rows = gb.x && tby == 1;
-            if (rows) k_phase<L, ST, CT, SP, false, false, true><<<gb, block_shape(), 0, stream>>>(a);
+            done = gb.x && tby == 1;
+            if (done) k_phase<L, ST, CT, SP, false, false, true><<<gb, block_shape(), 0, stream>>>(a);
         }
-        if (gb.x && !rows) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
+        if constexpr (kPairs) {
+            const PairGrid pg = pair_grid();
+            if (pg.total) k_phase_pair<L, ST, SP><<<pair_blocks(pg), BX * BY, 0, stream>>>(a, pg);
+            done = true;
+        }
+        if (gb.x && !done) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
         launch_check("k_phase");
         if (shell_blocks()) {
             launch_shell_pull(a);

@@ -41,10 +41,21 @@ def test_resume_is_bit_exact(tmp_path, text, precision):
 
 @pytest.mark.parametrize("precision", ["mixed_f16", "mixed_bf16"])
 def test_mixed_resume_is_close(tmp_path, precision):
-    """Checkpoints hold FP64 values; re-encoding a 16-bit deviation from its
-    decoded FP32 value could in principle move a small stored value by one
-    unit in the last place, so only closeness is required (the resumed runs
-    of these cases match bit for bit)."""
+    """Checkpoints hold FP64 values.  Re-encoding a 16-bit deviation from its
+    FP32 decode is not always idempotent: the decode s/2^10 + w phi rounds in
+    FP32.  So the restart's pulled buffer can differ from the saved one by one
+    16-bit ulp, and only closeness is required.
+
+    Measured for FP16 on this drop (density ratio 1000):
+    - right after loading, phi differs at 6 nodes by one FP32 ulp (6e-8),
+      while f and g match;
+    - the phi difference grows as 1.3e-8 (1 step), 4.1e-6 (10) and 1.7e-5
+      (40 steps);
+    - u, whose field is ~0 in this drop, differs by 1.5e-3 relative.
+
+    BF16 resumes bit for bit here.  A broken restart differs at O(1) from the
+    first step.  The bars (phi 1e-4, u 1e-2) stay below what separates FP16
+    from the FP64 oracle (2e-3, 5e-2)."""
     cfg = Config.from_string(cases.drop2d(n=24, radius=6))
     a = Solver(cfg, precision).init_fresh().step(30)
     d = tmp_path / "Checkpoint"
@@ -57,7 +68,7 @@ def test_mixed_resume_is_close(tmp_path, precision):
         x, y = b.interior(f), a.interior(f)
         err = np.linalg.norm(x - y) / np.linalg.norm(y)
         print(precision, f, "resume rel-L2", err, flush=True)
-        assert err <= 1e-5, (f, err)
+        assert err <= (1e-4 if f == "phi" else 1e-2), (f, err)
 
 
 def test_file_set_and_layout(tmp_path):

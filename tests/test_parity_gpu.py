@@ -175,7 +175,9 @@ def test_fp32_against_fp64_oracle(name, text, steps, tol):
 # Mixed precision: FP32 arithmetic, populations stored in 16 bits as scaled
 # deviations (g - w_q phi_prev) * 2^10 and f * 2^10, phi/u/p in FP32.  The
 # FP64 oracle is the yardstick; the tolerances below are the bar for this
-# storage format (FP16 keeps 11 significant bits, BF16 8).
+# storage format (FP16 keeps 11 significant bits, BF16 8).  The bulk runs on
+# x-adjacent node pairs: odd row lengths (33, 23 and 15 bulk nodes) end in a
+# half pair, and periodic x wraps split the pulled pair at both row ends.
 @pytest.mark.parametrize("precision,tol_phi,tol_u", [
     ("mixed_f16", 2e-3, 5e-2),
     ("mixed_bf16", 2e-2, 3e-1),
@@ -185,7 +187,11 @@ def test_fp32_against_fp64_oracle(name, text, steps, tol):
     ("bubble3d", cases.bubble3d(n=16, radius=4), 60),
     ("jet2d", cases.jet2d(24, 32, radius=4), 100),
     ("drop2d_200x40", cases.drop2d(n=200, ny=40, radius=12), 100),
-], ids=["drop2d", "bubble3d", "jet2d", "drop2d_200x40"])
+    ("drop2d_33", cases.drop2d(n=33, radius=8), 100),
+    ("box2d_25x20", cases.box2d([("x-", "wall"), ("x+", "do nothing"), ("y-", "free convect"),
+                                 ("y+", "periodic")], n=25, ny=20), 100),
+    ("bubble3d_15", cases.bubble3d(n=15, radius=4), 60),
+], ids=["drop2d", "bubble3d", "jet2d", "drop2d_200x40", "drop2d_33", "box2d_25x20", "bubble3d_15"])
 def test_mixed_against_fp64_oracle(precision, tol_phi, tol_u, name, text, steps):
     cfg, gpu, orc = run_pair(text, steps, precision=precision)
     a_phi, b_phi = gpu.interior("phi"), orc.interior("phi")
