@@ -39,10 +39,17 @@ struct Impl {
     // restored and the step kernels recompute the last macroscopic step
     // exactly; without them the reference's restart is reproduced
     // (f_post read directly, u_old = 0).
+    // `raw_f` / `raw_g` (16-bit storage only): the stored bytes of the pulled
+    // buffer, used instead of re-encoding f / g, so the restart is exact.
     virtual void resume(const std::vector<double>& f, const std::vector<double>& g, const std::vector<double>& fpost,
                         const std::vector<double>& gpost, const std::vector<double>* fprev,
                         const std::vector<double>* gprev, const std::vector<double>* phi_now,
-                        const std::vector<double>* phi_before, const std::vector<double>* uold) = 0;
+                        const std::vector<double>* phi_before, const std::vector<double>* uold,
+                        const std::vector<unsigned char>* raw_f = nullptr,
+                        const std::vector<unsigned char>* raw_g = nullptr) = 0;
+    // Stored bytes of the pulled buffer (f or g) of a 16-bit-storage handle
+    // that has stepped; false (and nothing written) otherwise.
+    virtual bool raw_pulled(bool is_f, std::vector<unsigned char>& out) = 0;
     // phi of the last macroscopic step and the one before, ghosts included.
     virtual void phi_pair(std::vector<double>& now, std::vector<double>& before) = 0;
     // f_prev / g_prev as CLIP stores them: f_post of the previous step on the

@@ -395,6 +395,8 @@ struct SimInfoFile {
 static_assert(sizeof(SimInfoFile) == 32, "simInfo layout");
 
 const char kStateMagic[8] = {'C', 'L', 'I', 'P', 'A', 'M', 'D', '2'};
+// optional trailing block of clip_amd_state.bin: raw 16-bit pulled populations
+const char kRawTag[8] = {'R', 'A', 'W', 'P', 'O', 'P', '1', '6'};
 
 }  // namespace
 
@@ -480,6 +482,16 @@ extern "C" int clip_checkpoint_save(clip_solver* s, const char* dir) {
         st.write(reinterpret_cast<const char*>(before.data()), (std::streamsize)(before.size() * sizeof(double)));
         s->impl->uold_soa(soa);
         st.write(reinterpret_cast<const char*>(soa.data()), (std::streamsize)(soa.size() * sizeof(double)));
+        // 16-bit storage: the stored bytes of the pulled f and g buffers, so
+        // that the restart does not re-encode decoded values
+        std::vector<unsigned char> rf, rg;
+        if (s->impl->raw_pulled(true, rf) && s->impl->raw_pulled(false, rg)) {
+            st.write(kRawTag, 8);
+            const long long bytes = (long long)rf.size();
+            st.write(reinterpret_cast<const char*>(&bytes), sizeof(bytes));
+            st.write(reinterpret_cast<const char*>(rf.data()), (std::streamsize)rf.size());
+            st.write(reinterpret_cast<const char*>(rg.data()), (std::streamsize)rg.size());
+        }
         if (!st) throw std::runtime_error("write failed: clip_amd_state.bin");
     });
 }
@@ -522,7 +534,8 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
         SimInfoFile ti;
         read_raw(d + "/timeInfo.bin", &ti, 1);
         std::vector<double> phi, phi_before, uold;
-        bool have_state = false;
+        std::vector<unsigned char> raw_f, raw_g;
+        bool have_state = false, have_raw = false;
         const std::string sp = d + "/clip_amd_state.bin";
         if (fs::exists(sp)) {
             std::ifstream st(sp, std::ios::binary);
@@ -541,10 +554,26 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
             st.read(reinterpret_cast<char*>(uold.data()), (std::streamsize)(uold.size() * sizeof(double)));
             if (!st) throw std::runtime_error("truncated clip_amd_state.bin");
             have_state = true;
+            char tag[8];
+            if (s->precision == CLIP_PREC_MIXED_F16 || s->precision == CLIP_PREC_MIXED_BF16) {
+                st.read(tag, 8);
+                if (st && std::memcmp(tag, kRawTag, 8) == 0) {
+                    long long bytes = 0;
+                    st.read(reinterpret_cast<char*>(&bytes), sizeof(bytes));
+                    if (!st || bytes != (long long)(nodes * Q * 2))
+                        throw std::runtime_error("clip_amd_state.bin: raw population block does not match");
+                    raw_f.resize((size_t)bytes);
+                    raw_g.resize((size_t)bytes);
+                    st.read(reinterpret_cast<char*>(raw_f.data()), bytes);
+                    st.read(reinterpret_cast<char*>(raw_g.data()), bytes);
+                    if (!st) throw std::runtime_error("truncated clip_amd_state.bin");
+                    have_raw = true;
+                }
+            }
         }
         s->impl->resume(f, g, fpost, gpost, have_prev ? &fprev : nullptr, have_prev ? &gprev : nullptr,
                         have_state ? &phi : nullptr, have_state ? &phi_before : nullptr,
-                        have_state ? &uold : nullptr);
+                        have_state ? &uold : nullptr, have_raw ? &raw_f : nullptr, have_raw ? &raw_g : nullptr);
         s->impl->step_count = ti.currentStep;
     });
     if (rc == CLIP_ERR_ARG || rc == CLIP_ERR_IO) {

@@ -657,10 +657,22 @@ struct SolverImpl : Impl {
         download(phib[1 - phc], (size_t)g.nodes, before);
     }
 
+    bool raw_pulled(bool is_f, std::vector<unsigned char>& out) override {
+        if constexpr (sizeof(ST) != 2) return false;
+        if (direct_old || !initialized) return false;
+        const size_t bytes = (size_t)g.nodes * QL * sizeof(ST);
+        out.resize(bytes);
+        HIP_CHECK(hipMemcpyAsync(out.data(), is_f ? fbuf[1 - cur] : gbuf[1 - cur], bytes, hipMemcpyDeviceToHost,
+                                 stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return true;
+    }
+
     void resume(const std::vector<double>& f, const std::vector<double>& gg, const std::vector<double>& fpost,
                 const std::vector<double>& gpost, const std::vector<double>* fprev, const std::vector<double>* gprev,
                 const std::vector<double>* phi_now, const std::vector<double>* phi_before,
-                const std::vector<double>* uold) override {
+                const std::vector<double>* uold, const std::vector<unsigned char>* raw_f,
+                const std::vector<unsigned char>* raw_g) override {
         const int tb = 256;
         if (phi_now && phi_before && uold) {
             // Exact restart: pop 0 = dev_f / dev_g (pulled) encoded relative to
@@ -669,10 +681,20 @@ struct SolverImpl : Impl {
             // step kernels recompute macroscopic() and the pending collision.
             upload(phib[0], *phi_before);
             upload(phib[1], *phi_now);
-            upload_pop(fbuf[0], f, nullptr);
-            upload_pop(gbuf[0], gg, phib[0]);
-            upload_pop(fbuf[1], f, nullptr);
-            upload_pop(gbuf[1], gg, phib[0]);
+            const size_t raw_bytes = (size_t)g.nodes * QL * sizeof(ST);
+            if (sizeof(ST) == 2 && raw_f && raw_g && raw_f->size() == raw_bytes && raw_g->size() == raw_bytes) {
+                // the stored 16-bit bytes themselves: no re-encoding
+                for (int b = 0; b < 2; ++b) {
+                    HIP_CHECK(hipMemcpyAsync(fbuf[b], raw_f->data(), raw_bytes, hipMemcpyHostToDevice, stream));
+                    HIP_CHECK(hipMemcpyAsync(gbuf[b], raw_g->data(), raw_bytes, hipMemcpyHostToDevice, stream));
+                }
+                HIP_CHECK(hipStreamSynchronize(stream));
+            } else {
+                upload_pop(fbuf[0], f, nullptr);
+                upload_pop(gbuf[0], gg, phib[0]);
+                upload_pop(fbuf[1], f, nullptr);
+                upload_pop(gbuf[1], gg, phib[0]);
+            }
             upload(u[0], *uold);
             for (int s = 0; s < 2 * L::D; ++s) {
                 if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;

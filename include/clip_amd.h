@@ -187,9 +187,10 @@ int clip_check_health(clip_solver* s, double* phi_sum);
  * `dir` (dev_f.bin, dev_g.bin, dev_f_post.bin, dev_g_post.bin,
  * [dev_f_prev.bin, dev_g_prev.bin], timeInfo.bin, domainSize.bin, info.txt)
  * plus clip_amd_state.bin (phi of the last two steps with their ghost rings
- * and u_old; the outflow planes go to dev_f_prev / dev_g_prev) so that a
- * resume is bit-exact.  Loading without the extension file reproduces the
- * reference's macroscopic() restart with u_old = 0. */
+ * and u_old; for 16-bit storage also the stored bytes of the pulled f and g;
+ * the outflow planes go to dev_f_prev / dev_g_prev) so that a resume is
+ * bit-exact.  Loading without the extension file reproduces the reference's
+ * macroscopic() restart with u_old = 0. */
 int clip_checkpoint_save(clip_solver* s, const char* dir);
 int clip_checkpoint_load(clip_solver* s, const char* dir);
 /* CheckPointer::rotateFolders (CheckPointer.cu:108-138). */
