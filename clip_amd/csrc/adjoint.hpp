@@ -208,7 +208,12 @@ template <class L, class CT>
 __global__ __launch_bounds__(ADJ_TB) void k_adj_pull(AdjArgs<CT> a, double* acc, int blocks) {
     constexpr int Q = L::Q, D = L::D;
     const DevGrid& g = a.g;
-    if (blockIdx.x == 0) accumulate4(a.part, blocks, acc);  // partials of k_adj_node
+    // the launch has one block more than the pull needs: that block folds
+    // k_adj_node's partials, next to the pull rather than after it
+    if (blockIdx.x == (unsigned)blocks) {
+        accumulate4(a.part, blocks, acc);
+        return;
+    }
     int c[3];
     if (!interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
     const long long n = g.idx(c[0], c[1], c[2]);

@@ -947,7 +947,7 @@ struct SolverImpl : Impl {
 
     // Reverse of the step from kept state `st` to the state whose phi is
     // `phin`.  The partial sums of the parameter adjoints are folded into
-    // the accumulator by k_adj_pull's first block.
+    // the accumulator by k_adj_pull's extra last block.
     void adjoint_step(CT* st, const CT* phin) {
         const AdjArgs<CT> A = adj_args(st, phin);
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
@@ -955,7 +955,7 @@ struct SolverImpl : Impl {
         launch_check("k_adj_node");
         k_adj_phi<L, CT, true><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_phi");
-        k_adj_pull<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
+        k_adj_pull<L, CT><<<blocks + 1, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
         launch_check("k_adj_pull");
     }
 
