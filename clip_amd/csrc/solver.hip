@@ -487,8 +487,8 @@ extern "C" int clip_checkpoint_save(clip_solver* s, const char* dir) {
         std::vector<unsigned char> rf, rg;
         if (s->impl->raw_pulled(true, rf) && s->impl->raw_pulled(false, rg)) {
             st.write(kRawTag, 8);
-            const long long bytes = (long long)rf.size();
-            st.write(reinterpret_cast<const char*>(&bytes), sizeof(bytes));
+            const long long head[2] = {(long long)s->precision, (long long)rf.size()};  // storage type, bytes
+            st.write(reinterpret_cast<const char*>(head), sizeof(head));
             st.write(reinterpret_cast<const char*>(rf.data()), (std::streamsize)rf.size());
             st.write(reinterpret_cast<const char*>(rg.data()), (std::streamsize)rg.size());
         }
@@ -558,16 +558,18 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
             if (s->precision == CLIP_PREC_MIXED_F16 || s->precision == CLIP_PREC_MIXED_BF16) {
                 st.read(tag, 8);
                 if (st && std::memcmp(tag, kRawTag, 8) == 0) {
-                    long long bytes = 0;
-                    st.read(reinterpret_cast<char*>(&bytes), sizeof(bytes));
-                    if (!st || bytes != (long long)(nodes * Q * 2))
+                    long long head[2] = {-1, 0};  // storage type, bytes
+                    st.read(reinterpret_cast<char*>(head), sizeof(head));
+                    if (!st || head[1] != (long long)(nodes * Q * 2))
                         throw std::runtime_error("clip_amd_state.bin: raw population block does not match");
-                    raw_f.resize((size_t)bytes);
-                    raw_g.resize((size_t)bytes);
-                    st.read(reinterpret_cast<char*>(raw_f.data()), bytes);
-                    st.read(reinterpret_cast<char*>(raw_g.data()), bytes);
+                    raw_f.resize((size_t)head[1]);
+                    raw_g.resize((size_t)head[1]);
+                    st.read(reinterpret_cast<char*>(raw_f.data()), head[1]);
+                    st.read(reinterpret_cast<char*>(raw_g.data()), head[1]);
                     if (!st) throw std::runtime_error("truncated clip_amd_state.bin");
-                    have_raw = true;
+                    // bytes of the other 16-bit type are not these populations:
+                    // then the decoded FP64 files are re-encoded instead
+                    have_raw = head[0] == (long long)s->precision;
                 }
             }
         }

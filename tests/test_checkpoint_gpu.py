@@ -60,6 +60,33 @@ def test_mixed_resume_is_bit_exact(tmp_path, precision):
         assert np.array_equal(x, y), f
 
 
+def test_mixed_state_block_checks(tmp_path):
+    """The raw 16-bit block of clip_amd_state.bin: a truncated block is an I/O
+    error; a block written by the other 16-bit type is not used (the decoded
+    files are re-encoded), so the resume is close to the run but need not be
+    exact."""
+    cfg = Config.from_string(cases.drop2d(n=24, radius=6))
+    a = Solver(cfg, "mixed_f16").init_fresh().step(30)
+    d = tmp_path / "Checkpoint"
+    a.checkpoint_save(d)
+    # another 16-bit type: loads, and stays close to the FP16 run
+    b = Solver(cfg, "mixed_bf16")
+    b.checkpoint_load(d)
+    a.step(20)
+    b.step(20)
+    x, y = b.interior("phi"), a.interior("phi")
+    err = np.linalg.norm(x - y) / np.linalg.norm(y)
+    print("bf16 from an fp16 checkpoint: phi rel-L2", err, flush=True)
+    assert err <= 2e-2
+    # truncated raw block
+    sp = d / "clip_amd_state.bin"
+    data = sp.read_bytes()
+    sp.write_bytes(data[:-100])
+    c = Solver(cfg, "mixed_f16")
+    with pytest.raises(ClipError):
+        c.checkpoint_load(d)
+
+
 def test_file_set_and_layout(tmp_path):
     cfg = Config.from_string(cases.jet2d(20, 24, radius=4))
     s = Solver(cfg, "f64").init_fresh().step(5)
