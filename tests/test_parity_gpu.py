@@ -191,7 +191,11 @@ def test_fp32_against_fp64_oracle(name, text, steps, tol):
     ("box2d_25x20", cases.box2d([("x-", "wall"), ("x+", "do nothing"), ("y-", "free convect"),
                                  ("y+", "periodic")], n=25, ny=20), 100),
     ("bubble3d_15", cases.bubble3d(n=15, radius=4), 60),
-], ids=["drop2d", "bubble3d", "jet2d", "drop2d_200x40", "drop2d_33", "box2d_25x20", "bubble3d_15"])
+    ("box3d_walls_15", cases.bubble3d(n=15, radius=3, rho_ratio=10, mu_ratio=10,
+                                      sides=[("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall"),
+                                             ("z-", "periodic"), ("z+", "periodic")]), 60),
+], ids=["drop2d", "bubble3d", "jet2d", "drop2d_200x40", "drop2d_33", "box2d_25x20", "bubble3d_15",
+        "box3d_walls_15"])
 def test_mixed_against_fp64_oracle(precision, tol_phi, tol_u, name, text, steps):
     cfg, gpu, orc = run_pair(text, steps, precision=precision)
     a_phi, b_phi = gpu.interior("phi"), orc.interior("phi")
