@@ -36,7 +36,7 @@ struct SolverImpl : Impl {
     static constexpr bool kRowAddressing = sizeof(CT) == 8;
     // 16-bit storage: the bulk phase sum on x-adjacent node pairs
     // (k_phase_pair), one 32-bit access per pulled g of both nodes.  Measured
-    // on the 256^3 bubble: 0.190 -> 0.164 ms.  The same pairing of k_fused
+    // on the 256^3 bubble: 0.190 -> 0.164-0.184 ms.  The same pairing of k_fused
     // (two nodes per lane: 192 VGPRs; or the f pair and the g pair on
     // adjacent lanes with a lane swap) was slower than one node per lane
     // (1.39 and 1.05 ms against 0.83 ms), so k_fused is not paired.
