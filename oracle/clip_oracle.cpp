@@ -27,6 +27,14 @@
 // Build: oracle/build.sh (g++ -O2 -fopenmp -shared).
 #include "clip_amd.h"
 
+// Planted mistakes for the mutation checks of the pin tests
+// (tests/test_oracle_mutants.py builds the oracle with -DCLIP_ORACLE_MUTANT=k
+// and expects the pin of the mutated function to fail).  0: none.
+#ifndef CLIP_ORACLE_MUTANT
+#define CLIP_ORACLE_MUTANT 0
+#endif
+#define MUTANT(k) (CLIP_ORACLE_MUTANT == (k))
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -407,8 +415,9 @@ void normals(Oracle& o) {
                 const size_t id = o.idx(i, j, k);
                 double s = 0;
                 for (int a = 0; a < o.D; ++a) s += o.dphi[id * o.D + a] * o.dphi[id * o.D + a];
-                const double t = std::sqrt(s) + 1e-32;
-                for (int a = 0; a < o.D; ++a) o.normal[id * o.D + a] = o.dphi[id * o.D + a] / t;
+                const double t = MUTANT(8) ? s + 1e-32 : std::sqrt(s) + 1e-32;
+                for (int a = 0; a < o.D; ++a)
+                    o.normal[id * o.D + a] = o.dphi[id * o.D + (MUTANT(9) ? (a + 1) % o.D : a)] / t;
             }
 }
 
@@ -578,9 +587,12 @@ void velocity_pass(Oracle& o) {
                             double corr = 0;
                             for (int t = 0; t < o.D; ++t)
                                 if (t != ax) corr += o.L.e[a][t] * Mt[t];
-                            o.f[gid * o.Q + a] = o.f[gid * o.Q + b] + feq[a] - feq[b] - (1.0 / o.L.weight) * corr;
+                            const double wt = MUTANT(2) ? o.L.weight + 1.0 : o.L.weight;
+                            const double sg = MUTANT(3) ? -1.0 : 1.0;
+                            o.f[gid * o.Q + a] = MUTANT(4) ? o.f[gid * o.Q + b] + feq[b] - feq[a] - (1.0 / wt) * corr
+                                                           : o.f[gid * o.Q + b] + feq[a] - feq[b] - sg * (1.0 / wt) * corr;
                             double gv = geq[a] - (o.g[gid * o.Q + b] - geq[b]);
-                            if (m != 0) gv -= N / o.L.weight;
+                            if (m != 0 || MUTANT(1)) gv -= N / o.L.weight;
                             o.g[gid * o.Q + a] = gv;
                         }
                     } else {
@@ -588,8 +600,9 @@ void velocity_pass(Oracle& o) {
                         bc[ax] = (s % 2 == 0) ? o.dmin[ax] : o.dmax[ax];
                         const size_t bid = o.idx(bc[0], bc[1], bc[2]);
                         for (int m = 0; m < o.L.nin; ++m) {
-                            o.g[gid * o.Q + in[m]] = o.g[bid * o.Q + o.L.opp[in[m]]];
-                            o.f[gid * o.Q + in[m]] = o.f[bid * o.Q + o.L.opp[in[m]]];
+                            const int src = MUTANT(5) ? in[m] : o.L.opp[in[m]];
+                            o.g[gid * o.Q + in[m]] = o.g[bid * o.Q + src];
+                            o.f[gid * o.Q + in[m]] = o.f[bid * o.Q + src];
                         }
                         o.phi[gid] = o.phi[bid];
                     }
@@ -810,7 +823,9 @@ void device_initializer(Oracle& o) {
                     double en = 0;
                     for (int d = 0; d < o.D; ++d) en += o.L.e[q][d] * nr[d];
                     const double hlp = o.L.w[q] * ((1.0 - 4.0 * ((c0 - 0.5) * (c0 - 0.5))) / o.c.interface_width * en);
-                    const double gv = c0 * (ga + o.L.w[q]) - 0.5 * hlp;
+                    const double gv = MUTANT(6)   ? c0 * (ga + o.L.w[q]) + 0.5 * hlp
+                                      : MUTANT(7) ? c0 * (ga + o.L.w[q]) - 0.5 * 3.0 * o.c.mobility * hlp
+                                                  : c0 * (ga + o.L.w[q]) - 0.5 * hlp;
                     const double fv = pp * o.L.w[q] + ga;
                     o.g[id * o.Q + q] = o.gpost[id * o.Q + q] = gv;
                     o.f[id * o.Q + q] = o.fpost[id * o.Q + q] = fv;

@@ -12,7 +12,9 @@ import numpy as np
 from clip_amd import _abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "oracle", "_build", "libclip_oracle.so")
+# CLIP_ORACLE_LIB selects another build of the same source (the mutation
+# checks of tests/test_oracle_mutants.py load planted-mistake builds this way)
+LIB = os.environ.get("CLIP_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "libclip_oracle.so")
 _lib = None
 
 

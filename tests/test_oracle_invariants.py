@@ -332,6 +332,137 @@ def test_velocity_inlet_returns_equilibrium_for_equilibrium_input():
         assert np.allclose(g2[1:-1, jg, 0, q], phi[1:-1, jg, 0, 0] * (Gm[q] + L["w"][q]), atol=1e-16)
 
 
+def _inlet_plane_nodes(cfg, side):
+    """Ghost nodes of `side`'s plane over the interior tangential range, with
+    the boundary node next to each and whether it lies inside geometry 0."""
+    lib = oracle_lib.load()
+    ex = cfg.extents()
+    dim, ax = cfg.dim, side // 2
+    ghost = ex[ax] - 1 if side % 2 else 0
+    bnd = ghost - 1 if side % 2 else 1
+    ranges = [range(1, ex[a] - 1) if a < dim else range(1) for a in range(3)]
+    ranges[ax] = [ghost]
+    out = []
+    for c in itertools.product(*ranges):
+        b = list(c)
+        b[ax] = bnd
+        z = float(c[2]) if dim == 3 else 0.0
+        out.append((c, tuple(b), lib.orc_sdf(cfg.raw, float(c[0]), float(c[1]), z) <= 0))
+    return out
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_velocity_inlet_invariants_on_random_ghosts(dim):
+    """JetBoundary (Solver.cu:578-633) on arbitrary (non-equilibrium) ghost
+    populations, inlet disc smaller than the plane.  Closed forms that follow
+    from the update rules, not from restating them:
+    inside the disc, with G = Gamma(U), geq = phi_ghost (G + w) and
+    M_t = sum_{q>=1, e_n=0} e_t (f_q - G_q) before the update,
+      * tangential and outgoing populations are untouched;
+      * sum_{q>=1} (g_q - geq_q) = 0 (the -N/weight terms cancel N exactly:
+        NIN - 1 = weight);
+      * the normal non-equilibrium momentum sum_q e_n (f_q - G_q) = 0;
+      * the tangential one sum_q e_t (f_q - G_q) = (1 - 2/weight) M_t:
+        0 in 2D (weight 2), M_t / 2 in 3D (weight 4), since two incoming
+        directions carry e_t = +-1;
+    outside the disc the incoming populations are the boundary node's
+    outgoing ones (bounce-back copy) and phi_ghost = phi_boundary."""
+    side = 3  # y+
+    if dim == 2:
+        cfg = Config.from_string(cases.jet2d(12, 10, radius=3.0))
+    else:
+        cfg = Config.from_string(cases.jet3d(n=(10, 12, 10), radius=3.0, inlet_side="y+"))
+    o = Oracle(cfg).init()
+    ex = cfg.extents()
+    L = lat(dim)
+    e, w, Q = L["e"][:, :dim], L["w"], cfg.q
+    opp = [int(np.where((e == -e[q]).all(axis=1))[0][0]) for q in range(Q)]
+    ax, inward = side // 2, (-1 if side % 2 else 1)
+    rng = np.random.default_rng(11)
+    f0 = 0.1 * w + rng.uniform(-0.05, 0.05, size=(*ex, Q))
+    g0 = rng.uniform(0.0, 0.1, size=(*ex, Q))
+    phi0 = rng.uniform(0.1, 0.9, size=(*ex, 1))
+    o.set("f", f0)
+    o.set("g", g0)
+    o.set("phi", phi0)
+    o.phase("velocity")
+    f1, g1, phi1 = o.get("f"), o.get("g"), o.get("phi")
+    U = np.array(cfg.raw.bc_value[side][:dim])
+    G = np.array([oracle_lib.gamma(dim, q, U) for q in range(Q)])
+    incoming = [q for q in range(Q) if e[q, ax] == inward]
+    tang = [a for a in range(dim) if a != ax]
+    weight = len(incoming) - 1
+    n_in = n_out = 0
+    for c, b, inside in _inlet_plane_nodes(cfg, side):
+        if inside:
+            n_in += 1
+            for q in range(Q):
+                if q not in incoming:
+                    assert f1[c][q] == f0[c][q] and g1[c][q] == g0[c][q]
+            geq = phi0[c][0] * (G + w)
+            assert abs((g1[c][1:] - geq[1:]).sum()) < 1e-15
+            assert abs(((f1[c] - G) * e[:, ax]).sum()) < 1e-15
+            for t in tang:
+                Mt = sum(e[q, t] * (f0[c][q] - G[q]) for q in range(1, Q) if e[q, ax] == 0)
+                after = ((f1[c] - G) * e[:, t]).sum()
+                assert abs(after - (1.0 - 2.0 / weight) * Mt) < 1e-15, (c, t)
+            assert phi1[c][0] == phi0[c][0]
+        else:
+            n_out += 1
+            for q in incoming:
+                assert f1[c][q] == f0[b][opp[q]] and g1[c][q] == g0[b][opp[q]]
+            assert phi1[c][0] == phi0[b][0]
+    assert n_in > 0 and n_out > 0
+
+
+@pytest.mark.parametrize("text", [cases.drop2d(n=24, radius=6), cases.bubble3d(n=10, radius=3)])
+def test_initial_g_moments(text):
+    """KernelInitializeDistributions (NsAllen.cu:183-195):
+    g = phi (Gamma(0) + w) - 1/2 w (1 - 4 (phi - 1/2)^2) / W (e . n), so by the
+    lattice moments sum_q w = 1, sum_q w e = 0, sum_q w e e = I/3:
+    sum_q g = phi and sum_q e g = -(1 - 4 (phi - 1/2)^2) n / (6 W)."""
+    cfg = Config.from_string(text)
+    o = Oracle(cfg).init()
+    sl = cfg.interior_slices()
+    L = lat(cfg.dim)
+    e = L["e"][:, :cfg.dim].astype(float)
+    g, phi, nr = o.get("g")[sl], o.get("phi")[sl][..., 0], o.get("normal")[sl]
+    assert np.abs(np.linalg.norm(nr, axis=-1) - 1).max() < 1e-12  # the band matters: n is a unit vector
+    assert np.allclose(g.sum(-1), phi, rtol=0, atol=1e-15)
+    want = -(1 - 4 * (phi - 0.5) ** 2)[..., None] * nr / (6 * cfg.interface_width)
+    got = np.einsum("...q,qa->...a", g, e)
+    assert np.abs(want).max() > 1e-3
+    assert np.allclose(got, want, rtol=0, atol=1e-16)
+
+
+@pytest.mark.parametrize("text", [cases.drop2d(n=12, radius=3), cases.bubble3d(n=8, radius=2)])
+def test_normal_is_unit_gradient_direction(text):
+    """normal_FD (NsAllen.cu:241-275): on a linear phi (where the isotropic
+    gradient is exact) n = grad phi / |grad phi| at every interior node, a
+    unit vector; on a uniform phi n = 0 (the 1e-32 guard); ghost normals stay
+    at their initial 0."""
+    cfg = Config.from_string(text)
+    o = Oracle(cfg).init()
+    D = cfg.dim
+    grad = np.array([0.013, -0.021, 0.007])[:D]
+    if D == 2:
+        grad = np.array([0.013, -0.021])
+    _set_phi(o, lambda x, y, z: 0.3 + grad[0] * x + grad[1] * y + (grad[2] * z if D == 3 else 0.0))
+    o.phase("gradient", "normals")
+    nr = o.interior("normal")
+    want = grad / np.linalg.norm(grad)
+    for d in range(D):
+        assert np.allclose(nr[..., d], want[d], rtol=0, atol=1e-14)
+    assert np.allclose(np.linalg.norm(nr, axis=-1), 1.0, rtol=0, atol=1e-14)
+    full = o.get("normal")
+    ring = np.ones(full.shape[:3], dtype=bool)
+    ring[cfg.interior_slices()] = False
+    assert np.all(full[ring] == 0.0)
+    _set_phi(o, lambda x, y, z: 0.25 + 0.0 * x)
+    o.phase("gradient", "normals")
+    assert np.all(o.interior("normal") == 0.0)
+
+
 def test_mirror_copies_boundary_phi():
     """kernelMirrorBoundary (Solver.cu:927-978)."""
     cfg = Config.from_string(cases.box2d([("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall")], n=6, ny=5))
