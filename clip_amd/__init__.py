@@ -81,14 +81,25 @@ class Config:
 
     def with_params(self, sigma=None, mobility=None):
         """Copy with the surface tension and/or the mobility replaced
-        (derived beta, kappa follow: InputData.cu:76-114)."""
+        (derived beta, kappa follow: InputData.cu:76-114).  Both are kept as
+        explicit overrides, so a case that derives them (RTI derives the
+        mobility from Pe) uses the values given."""
         c = self.copy()
         if sigma is not None:
             c.raw.has_sigma = 1
             c.raw.sigma_override = float(sigma)
         if mobility is not None:
-            c.raw.mobility = float(mobility)
+            c.raw.has_mobility_override = 1
+            c.raw.mobility_override = float(mobility)
         return c.derive()
+
+    def sdf(self, x, y, z=0.0, geometry_id=0):
+        """Geometry::sdf (Geometry.cuh:102-161): signed distance to the first
+        geometry object with this id (positive outside; 1e10 if none)."""
+        out = ctypes.c_double()
+        _abi.check(_abi.load().clip_geometry_sdf(ctypes.byref(self.raw), int(geometry_id), float(x), float(y),
+                                                 float(z), ctypes.byref(out)))
+        return out.value
 
     def interior_shape(self):
         ex = self.extents()
@@ -180,10 +191,26 @@ class Solver:
         return out
 
     def set(self, field, values, layout="grid"):
+        """Overwrites one of the reference's device arrays (clip_set_field):
+        phi, u, p, f, g, f_post or g_post, ghost cells included, in the same
+        shape get() returns.  The next step collides from them; call
+        macroscopic() to recompute phi, p and u from f_post / g_post."""
+        if field not in ("phi", "u", "p", "f", "g", "f_post", "g_post"):
+            raise ValueError(f"{field!r} cannot be set (rho, mu, dphi and normal are derived from phi)")
         arr = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        ex = self.config.extents()
+        want = ex[0] * ex[1] * ex[2] * self.ndof(field)
+        if arr.size != want:
+            raise ValueError(f"{field}: {arr.size} values given, the grid holds {want} (ghost cells included)")
         lay = 1 if layout == "soa" else 0
         _abi.check(self._lib.clip_set_field(self._h, FIELD[field], lay,
                                             arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def macroscopic(self):
+        """NSAllen::macroscopic: phi, mu, grad phi, p and u recomputed from the
+        current f_post / g_post (u_old = the current u)."""
+        _abi.check(self._lib.clip_macroscopic(self._h))
+        return self
 
     def interior(self, field):
         a = self.get(field)

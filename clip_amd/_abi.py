@@ -76,6 +76,8 @@ class ClipConfig(ctypes.Structure):
         ("inlet_modes", ctypes.c_int),
         ("inlet_mode_amp", ctypes.c_double * MAX_INLET_MODES),
         ("inlet_mode_phase", ctypes.c_double * MAX_INLET_MODES),
+        ("has_mobility_override", ctypes.c_int),
+        ("mobility_override", ctypes.c_double),
     ]
 
 
@@ -106,6 +108,9 @@ SIGNATURES = {
     "clip_synchronize": (ctypes.c_int, [_P]),
     "clip_get_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
     "clip_set_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    "clip_macroscopic": (ctypes.c_int, [_P]),
+    "clip_geometry_sdf": (ctypes.c_int, [ctypes.POINTER(ClipConfig), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _D]),
     "clip_sum": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "clip_check_health": (ctypes.c_int, [_P, _D]),
     "clip_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_char_p]),

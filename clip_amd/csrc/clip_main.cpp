@@ -41,6 +41,7 @@ int main(int argc, char** argv) {
         clip::VTSwriter vtsWriter(ns, inputData, timeInfo, "results", "results");
         if (resume) {
             checkpointer.load();
+            ns.macroscopic();  // main.cu:67-71 (already part of the load: skipped)
         } else {
             ns.initialCondition();
             ns.deviceInitializer();

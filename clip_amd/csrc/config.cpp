@@ -393,6 +393,7 @@ void derive(clip_config* c) {
         c->sigma = (c->rho_h * c->reference_velocity * c->reference_velocity * L) / c->We;
         c->mu_h = (c->rho_h * c->reference_velocity * L) / c->Re;
     }
+    if (c->has_mobility_override) c->mobility = c->mobility_override;
     if (c->has_sigma) c->sigma = c->sigma_override;
     if (c->has_mu_h) c->mu_h = c->mu_h_override;
     c->mu_l = c->mu_h / c->mu_ratio;

@@ -47,8 +47,22 @@ struct Impl {
                         const std::vector<double>* phi_before, const std::vector<double>* uold,
                         const std::vector<unsigned char>* raw_f = nullptr,
                         const std::vector<unsigned char>* raw_g = nullptr) = 0;
-    // Stored bytes of the pulled buffer (f or g) of a 16-bit-storage handle
-    // that has stepped; false (and nothing written) otherwise.
+    // Restart into the direct state: the reference's arrays as they stand
+    // between two solve() calls (f_post / g_post held directly).  `phi_ref`
+    // is the phi the raw 16-bit bytes are encoded against (unless ref_zero).
+    virtual void resume_direct(const std::vector<double>& f, const std::vector<double>& g,
+                               const std::vector<double>& fpost, const std::vector<double>& gpost,
+                               const std::vector<double>& phi_now, const std::vector<double>& phi_ref,
+                               const std::vector<double>& u, const std::vector<double>& p, bool ref_zero,
+                               const std::vector<unsigned char>* raw_f, const std::vector<unsigned char>* raw_g) = 0;
+    // phi, the phi the pulled buffer is encoded against, u and p when the
+    // handle is in the direct state; false otherwise.
+    virtual bool direct_state(std::vector<double>& phi_now, std::vector<double>& phi_ref, std::vector<double>& u,
+                              std::vector<double>& p) = 0;
+    // NSAllen::macroscopic (NsAllen.cu:866-877) on the current f_post / g_post.
+    virtual void macroscopic() = 0;
+    // Stored bytes of the pulled (or, in the direct state, the f_post / g_post)
+    // buffer of a 16-bit-storage handle; false (and nothing written) otherwise.
     virtual bool raw_pulled(bool is_f, std::vector<unsigned char>& out) = 0;
     // phi of the last macroscopic step and the one before, ghosts included.
     virtual void phi_pair(std::vector<double>& now, std::vector<double>& before) = 0;
@@ -72,7 +86,8 @@ struct Impl {
     unsigned long long bytes = 0;
     long long step_count = 0;
     bool initialized = false;
-    bool direct_old = false;  // pop[1-cur] holds f_post directly
+    bool direct_old = false;    // pop[1-cur] holds f_post directly
+    bool old_ref_zero = false;  // pop[1-cur] stored without phi reference
     std::vector<double> direct_f_interior, direct_g_interior;  // dev_f/dev_g interior after a resume
 };
 

@@ -141,7 +141,8 @@ struct StepArgs {
     ST* gout;
     CT* phi;            // new phi with ghost ring (k_phase writes, k_fused reads the stencil)
     const CT* phi_ref;  // phi the pulled buffer was encoded with (nullptr = 0)
-    const CT* uin;   // u of the previous macroscopic step (D planes); outflow fix-ups read it
+    const CT* uin;   // u of the previous macroscopic step (D planes): u_old of the velocity
+    const CT* ufc;   // u the outflow fix-ups of the pull read (the u of the streaming step's start)
     CT* uout;        // u of this macroscopic step
     const CT* ucol;  // u the collide-only mode collides with
     CT* p;           // pressure (read by the collide-only mode; written by exports)
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
             sum += SP::ld(a.gin, (long long)q * a.g.nodes + src, q, r);
         }
     } else {
-        Puller<L, ST, CT, SP, true> P{a.gin, a.g, a.bc, a.uin, a.prev_g_in, a.phi_ref};
+        Puller<L, ST, CT, SP, true> P{a.gin, a.g, a.bc, a.ufc, a.prev_g_in, a.phi_ref};
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) sum += P.lvl3(c, q);
     }
@@ -727,7 +728,7 @@ __global__ __launch_bounds__(BX* BY) void k_shell_pull(StepArgs<ST, CT> a) {
 #define CLIP_SHELL_DIR(k)                                                                                   \
     case k:                                                                                                 \
         if constexpr (k < L::Q)                                                                             \
-            shell_pull_dir<L, ST, CT, SP, k>(a.fin, a.gin, a.g, a.bc, a.uin, pfi, pgi, a.phi_ref, a.shell_f, \
+            shell_pull_dir<L, ST, CT, SP, k>(a.fin, a.gin, a.g, a.bc, a.ufc, pfi, pgi, a.phi_ref, a.shell_f, \
                                              a.shell_g, pfo, pgo, t, S, c);                                 \
         break;
     switch (blockIdx.y) {
@@ -1330,8 +1331,8 @@ __global__ void k_export_post(StepArgs<ST, CT> a, bool is_f, CT* out, CT* pout) 
         CT v;
         if (!interior) v = SP::ld(src, (long long)q * g.nodes + n, q, CT(0));
         else if (DIRECT) v = SP::ld(src, (long long)q * g.nodes + n, q, gref<SP>(ref, ref_index<L>(g, a.bc, c, q)));
-        else if (is_f) v = Puller<L, ST, CT, SP, false>{src, g, a.bc, a.uin, prev, ref}.lvl3(c, q);
-        else v = Puller<L, ST, CT, SP, true>{src, g, a.bc, a.uin, prev, ref}.lvl3(c, q);
+        else if (is_f) v = Puller<L, ST, CT, SP, false>{src, g, a.bc, a.ufc, prev, ref}.lvl3(c, q);
+        else v = Puller<L, ST, CT, SP, true>{src, g, a.bc, a.ufc, prev, ref}.lvl3(c, q);
         out[(long long)q * g.nodes + n] = v;
         sum += v;
     }

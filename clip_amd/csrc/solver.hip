@@ -4,6 +4,7 @@
 // precision in inst_f64.hip / inst_f32.hip.
 #include "clip_amd.h"
 #include "common.hpp"
+#include "geometry.hpp"
 #include "impl.hpp"
 
 #include <hip/hip_runtime.h>
@@ -198,6 +199,9 @@ int clip_set_field(clip_solver* s, int field, int layout, const double* in) {
     if (!s || !in) return set_error(CLIP_ERR_ARG, "null argument");
     const int nd = ndof_of(s, field);
     if (nd < 0) return set_error(CLIP_ERR_ARG, "unknown field");
+    if (field == CLIP_FIELD_RHO || field == CLIP_FIELD_MU || field == CLIP_FIELD_DPHI || field == CLIP_FIELD_NORMAL)
+        return set_error(CLIP_ERR_ARG, "RHO, MU, DPHI and NORMAL are derived from PHI and cannot be set");
+    if (layout != CLIP_LAYOUT_AOS && layout != CLIP_LAYOUT_SOA) return set_error(CLIP_ERR_ARG, "unknown layout");
     if (!s->impl->initialized) return set_error(CLIP_ERR_STATE, "solver not initialised");
     return guard([&] {
         std::vector<double> soa;
@@ -207,6 +211,20 @@ int clip_set_field(clip_solver* s, int field, int layout, const double* in) {
         else aos_to_soa(in, soa, e, nd);
         s->impl->set_field(field, soa);
     });
+}
+
+int clip_macroscopic(clip_solver* s) {
+    if (!s) return set_error(CLIP_ERR_ARG, "null handle");
+    if (!s->impl->initialized) return set_error(CLIP_ERR_STATE, "solver not initialised");
+    return guard([&] { s->impl->macroscopic(); });
+}
+
+int clip_geometry_sdf(const clip_config* cfg, int id, double x, double y, double z, double* out) {
+    if (!cfg || !out) return set_error(CLIP_ERR_ARG, "null argument");
+    if (cfg->num_geometries < 0 || cfg->num_geometries > CLIP_MAX_GEOMETRIES)
+        return set_error(CLIP_ERR_ARG, "num_geometries out of range");
+    *out = clipamd::sdf(*cfg, id, x, y, z);
+    return CLIP_OK;
 }
 
 int clip_sum(clip_solver* s, int field, double* out) {
@@ -394,7 +412,9 @@ struct SimInfoFile {
 };
 static_assert(sizeof(SimInfoFile) == 32, "simInfo layout");
 
-const char kStateMagic[8] = {'C', 'L', 'I', 'P', 'A', 'M', 'D', '2'};
+// version 2: pulled state only; version 3 adds the mode word (0 pulled, 1 direct)
+const char kStateMagic2[8] = {'C', 'L', 'I', 'P', 'A', 'M', 'D', '2'};
+const char kStateMagic[8] = {'C', 'L', 'I', 'P', 'A', 'M', 'D', '3'};
 // optional trailing block of clip_amd_state.bin: raw 16-bit pulled populations
 const char kRawTag[8] = {'R', 'A', 'W', 'P', 'O', 'P', '1', '6'};
 
@@ -468,20 +488,37 @@ extern "C" int clip_checkpoint_save(clip_solver* s, const char* dir) {
         write_raw(d + "/timeInfo.bin", &ti, 1);
         std::vector<double> N(s->cfg.N, s->cfg.N + 3);
         write_field(d + "/domainSize.bin", N, single);
-        // Extension (no reference counterpart): phi of the last and the previous
-        // macroscopic step with their ghost rings, and the u the last
-        // macroscopic step started from, for a bit-exact resume.
+        // Extension (no reference counterpart).  Pulled state (mode 0): phi of
+        // the last and the previous macroscopic step with their ghost rings,
+        // and the u the last macroscopic step started from, for a bit-exact
+        // resume.  Direct state (mode 1, right after initialisation, a field
+        // upload or macroscopic()): phi, the phi the stored f_post / g_post
+        // are encoded against, u, p and whether that encoding is absolute.
         std::ofstream st(d + "/clip_amd_state.bin", std::ios::binary);
         if (!st.is_open()) throw std::runtime_error("Failed to open " + d + "/clip_amd_state.bin");
+        std::vector<double> now, ref, uu, pp;
+        const bool direct = s->impl->direct_state(now, ref, uu, pp);
         st.write(kStateMagic, 8);
-        const long long hdr[4] = {D, Q, (long long)nodes, s->impl->step_count};
+        const long long hdr[5] = {D, Q, (long long)nodes, s->impl->step_count, direct ? 1 : 0};
         st.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
-        std::vector<double> before;
-        s->impl->phi_pair(soa, before);
-        st.write(reinterpret_cast<const char*>(soa.data()), (std::streamsize)(soa.size() * sizeof(double)));
-        st.write(reinterpret_cast<const char*>(before.data()), (std::streamsize)(before.size() * sizeof(double)));
-        s->impl->uold_soa(soa);
-        st.write(reinterpret_cast<const char*>(soa.data()), (std::streamsize)(soa.size() * sizeof(double)));
+        auto put = [&](const std::vector<double>& v) {
+            st.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
+        };
+        if (direct) {
+            put(now);
+            put(ref);
+            put(uu);
+            put(pp);
+            const long long zero = s->impl->old_ref_zero ? 1 : 0;
+            st.write(reinterpret_cast<const char*>(&zero), sizeof(zero));
+        } else {
+            std::vector<double> before;
+            s->impl->phi_pair(soa, before);
+            put(soa);
+            put(before);
+            s->impl->uold_soa(soa);
+            put(soa);
+        }
         // 16-bit storage: the stored bytes of the pulled f and g buffers, so
         // that the restart does not re-encode decoded values
         std::vector<unsigned char> rf, rg;
@@ -533,25 +570,35 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
         }
         SimInfoFile ti;
         read_raw(d + "/timeInfo.bin", &ti, 1);
-        std::vector<double> phi, phi_before, uold;
+        std::vector<double> phi, phi_before, uold, pp;
         std::vector<unsigned char> raw_f, raw_g;
-        bool have_state = false, have_raw = false;
+        bool have_state = false, have_raw = false, direct = false, ref_zero = true;
         const std::string sp = d + "/clip_amd_state.bin";
         if (fs::exists(sp)) {
             std::ifstream st(sp, std::ios::binary);
             char magic[8];
-            long long hdr[4];
+            long long hdr[5] = {0, 0, 0, 0, 0};
             st.read(magic, 8);
-            st.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
-            if (!st || std::memcmp(magic, kStateMagic, 8) != 0 || hdr[0] != D || hdr[1] != Q ||
-                hdr[2] != (long long)nodes)
+            const bool v3 = st && std::memcmp(magic, kStateMagic, 8) == 0;
+            const bool v2 = st && std::memcmp(magic, kStateMagic2, 8) == 0;
+            st.read(reinterpret_cast<char*>(hdr), (std::streamsize)((v3 ? 5 : 4) * sizeof(long long)));
+            if (!st || !(v2 || v3) || hdr[0] != D || hdr[1] != Q || hdr[2] != (long long)nodes || hdr[4] < 0 ||
+                hdr[4] > 1)
                 throw std::runtime_error("clip_amd_state.bin does not match this configuration");
-            phi.resize(nodes);
-            phi_before.resize(nodes);
-            uold.resize(nodes * D);
-            st.read(reinterpret_cast<char*>(phi.data()), (std::streamsize)(phi.size() * sizeof(double)));
-            st.read(reinterpret_cast<char*>(phi_before.data()), (std::streamsize)(phi_before.size() * sizeof(double)));
-            st.read(reinterpret_cast<char*>(uold.data()), (std::streamsize)(uold.size() * sizeof(double)));
+            direct = hdr[4] == 1;
+            auto get = [&](std::vector<double>& v, size_t n) {
+                v.resize(n);
+                st.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(double)));
+            };
+            get(phi, nodes);
+            get(phi_before, nodes);
+            get(uold, nodes * D);
+            if (direct) {
+                get(pp, nodes);
+                long long zero = 1;
+                st.read(reinterpret_cast<char*>(&zero), sizeof(zero));
+                ref_zero = zero != 0;
+            }
             if (!st) throw std::runtime_error("truncated clip_amd_state.bin");
             have_state = true;
             char tag[8];
@@ -573,9 +620,13 @@ extern "C" int clip_checkpoint_load(clip_solver* s, const char* dir) {
                 }
             }
         }
-        s->impl->resume(f, g, fpost, gpost, have_prev ? &fprev : nullptr, have_prev ? &gprev : nullptr,
-                        have_state ? &phi : nullptr, have_state ? &phi_before : nullptr,
-                        have_state ? &uold : nullptr, have_raw ? &raw_f : nullptr, have_raw ? &raw_g : nullptr);
+        if (have_state && direct)
+            s->impl->resume_direct(f, g, fpost, gpost, phi, phi_before, uold, pp, ref_zero,
+                                   have_raw ? &raw_f : nullptr, have_raw ? &raw_g : nullptr);
+        else
+            s->impl->resume(f, g, fpost, gpost, have_prev ? &fprev : nullptr, have_prev ? &gprev : nullptr,
+                            have_state ? &phi : nullptr, have_state ? &phi_before : nullptr,
+                            have_state ? &uold : nullptr, have_raw ? &raw_f : nullptr, have_raw ? &raw_g : nullptr);
         s->impl->step_count = ti.currentStep;
     });
     if (rc == CLIP_ERR_ARG || rc == CLIP_ERR_IO) {

@@ -63,7 +63,6 @@ struct SolverImpl : Impl {
     double* red_out = nullptr;
     int cur = 1, ucur = 0, pcur = 0, phc = 0;
     unsigned tbx = BX, tby = BY;  // block shape of the bulk launches (tbx * tby = BX * BY)
-    bool old_ref_zero = false;  // pop[1-cur] encoded without phi reference (reference restart)
     bool timing_on = false;
     hipEvent_t ev[8] = {};
     double t_total = 0, t_ghost = 0, t_phase = 0, t_fused = 0;
@@ -239,6 +238,7 @@ struct SolverImpl : Impl {
         a.phi = phi_new;
         a.phi_ref = phi_ref;
         a.uin = u[uin];
+        a.ufc = u[uin];
         a.uout = u[1 - uin];
         a.ucol = u[uin];
         a.p = p;
@@ -367,6 +367,37 @@ struct SolverImpl : Impl {
         }
     }
 
+    // Incoming populations of the ghost cells inside each inlet disc
+    // (JetBoundary, Solver.cu:578-619) from the tangential and outgoing ghost
+    // populations of pop 0 and phi at the ghost cell, into both buffers.  Its
+    // inputs are ghost values no phase of a step rewrites, so it runs at
+    // initialisation and after an upload of populations or phi.
+    void inlet_fill(const CT* phi) {
+        const int tb = 256;
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (cfg.bc_type[s] != CLIP_BC_VELOCITY) continue;
+            const long long ps = plane_size(g, s);
+            std::vector<CT> U(3 * ps);
+            const int ax = s / 2;
+            const int t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
+            for (long long t = 0; t < ps; ++t) {
+                int cc[3];
+                cc[t1] = (int)(t % g.ext[t1]);
+                cc[t2] = (int)(t / g.ext[t1]);
+                cc[ax] = (s & 1) ? g.dmax[ax] + 1 : g.dmin[ax] - 1;
+                double Uv[3];
+                inlet_velocity(cfg, s, cc, Uv);
+                for (int d = 0; d < 3; ++d) U[3 * t + d] = (CT)Uv[d];
+            }
+            CT* dU = scratch;
+            HIP_CHECK(hipMemcpyAsync(dU, U.data(), U.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
+            k_inlet_init<L, ST, CT, SP><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(
+                g, vflag[s], s, phi, fbuf[0], gbuf[0], fbuf[1], gbuf[1], dU);
+            launch_check("k_inlet_init");
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+
     void init_fresh() override {
         // Host phi over every node including ghosts (NsAllen.cu:768-824).
         std::vector<CT> h(g.nodes);
@@ -387,29 +418,7 @@ struct SolverImpl : Impl {
         k_init<L, ST, CT, SP><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(
             a, fbuf[1], gbuf[1], (CT)(cfg.reference_length / 2.0), (CT)cfg.sigma);
         launch_check("k_init");
-        // Time-invariant inlet populations inside the disc, side by side.
-        for (int s = 0; s < 2 * L::D; ++s) {
-            if (cfg.bc_type[s] != CLIP_BC_VELOCITY) continue;
-            const long long ps = plane_size(g, s);
-            std::vector<CT> U(3 * ps);
-            const int ax = s / 2;
-            const int t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
-            for (long long t = 0; t < ps; ++t) {
-                int cc[3];
-                cc[t1] = (int)(t % g.ext[t1]);
-                cc[t2] = (int)(t / g.ext[t1]);
-                cc[ax] = (s & 1) ? g.dmax[ax] + 1 : g.dmin[ax] - 1;
-                double Uv[3];
-                inlet_velocity(cfg, s, cc, Uv);
-                for (int d = 0; d < 3; ++d) U[3 * t + d] = (CT)Uv[d];
-            }
-            CT* dU = scratch;
-            HIP_CHECK(hipMemcpyAsync(dU, U.data(), U.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
-            k_inlet_init<L, ST, CT, SP><<<(unsigned)((ps + tb - 1) / tb), tb, 0, stream>>>(
-                g, vflag[s], s, phib[0], fbuf[0], gbuf[0], fbuf[1], gbuf[1], dU);
-            launch_check("k_inlet_init");
-            HIP_CHECK(hipStreamSynchronize(stream));
-        }
+        inlet_fill(phib[0]);
         // First collision straight from the initial f_post = f (pop 0).
         StepArgs<ST, CT> c = args(0, 0, 1, phib[0], phib[0]);
         k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(c);
@@ -589,15 +598,184 @@ struct SolverImpl : Impl {
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
+    // Replaces the interior of a population array (SoA, ghosts included) by
+    // that of `src`.
+    void replace_interior(std::vector<double>& dst, const std::vector<double>& src) const {
+        for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
+            for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
+                for (int i = g.dmin[0]; i <= g.dmax[0]; ++i)
+                    for (int q = 0; q < QL; ++q) {
+                        const size_t t = (size_t)q * g.nodes + g.idx(i, j, k);
+                        dst[t] = src[t];
+                    }
+    }
+
+    // Direct state: the handle holds the reference's device arrays as they
+    // stand between two solve() calls.  pop[0] = dev_f with its interior
+    // replaced by f_post (stored absolutely, read in place), pop[1] = dev_f
+    // (its ghost cells are what later pulls read; the collision rewrites the
+    // interior); the same for g.  p, u[ucur] and phib[phc] are dev_p, dev_vel
+    // and dev_c; the pending collision is recomputed from them.
+    void set_direct(const std::vector<double>& fpost, const std::vector<double>& gpost, const std::vector<double>& df,
+                    const std::vector<double>& dg) {
+        std::vector<double> a0 = df, b0 = dg;
+        replace_interior(a0, fpost);
+        replace_interior(b0, gpost);
+        upload_pop(fbuf[0], a0, nullptr);
+        upload_pop(gbuf[0], b0, nullptr);
+        upload_pop(fbuf[1], df, nullptr);
+        upload_pop(gbuf[1], dg, nullptr);
+        cur = 1;
+        direct_old = true;
+        old_ref_zero = true;
+        direct_f_interior = df;
+        direct_g_interior = dg;
+    }
+
+    // Moves the handle into the direct state with absolutely stored f_post /
+    // g_post (exact for FP64 / FP32 storage; 16-bit storage re-encodes the
+    // pending f_post / g_post once without the phi reference).
+    void materialise() {
+        if (direct_old && old_ref_zero) return;
+        std::vector<double> fpost, gpost, df, dg;
+        get_field(CLIP_FIELD_F_POST, fpost);  // first: refreshes p = sum f_post
+        get_field(CLIP_FIELD_G_POST, gpost);
+        get_field(CLIP_FIELD_F, df);
+        get_field(CLIP_FIELD_G, dg);
+        set_direct(fpost, gpost, df, dg);
+    }
+
+    // Writes one of the reference's device arrays.  The arrays keep the
+    // reference's roles: dev_f / dev_g are overwritten by the next collision
+    // except for their ghost cells, which later pulls read; f_post / g_post,
+    // p, u and phi feed the next collision (whose mu, grad phi and normal
+    // are recomputed from the phi given).  Nothing else is recomputed:
+    // clip_macroscopic does that, as NSAllen::macroscopic does.
     void set_field(int field, const std::vector<double>& in) override {
+        if (field != CLIP_FIELD_PHI && field != CLIP_FIELD_U && field != CLIP_FIELD_P && field != CLIP_FIELD_F &&
+            field != CLIP_FIELD_G && field != CLIP_FIELD_F_POST && field != CLIP_FIELD_G_POST)
+            throw ConfigError("field cannot be set (derived from phi)");
+        materialise();
         switch (field) {
-            case CLIP_FIELD_PHI: upload(phib[phc], in); break;
+            case CLIP_FIELD_PHI:
+                // Both phi buffers: ghost cells that no step rewrites (inside an
+                // inlet disc, sides without a mirror) are read from whichever
+                // buffer is current.  The direct state stores f_post without a
+                // phi reference, so the other buffer is free.
+                upload(phib[phc], in);
+                HIP_CHECK(hipMemcpyAsync(phib[1 - phc], phib[phc], (size_t)g.nodes * sizeof(CT),
+                                         hipMemcpyDeviceToDevice, stream));
+                break;
             case CLIP_FIELD_U: upload(u[ucur], in); break;
             case CLIP_FIELD_P: upload(p, in); break;
-            default: throw ConfigError("field cannot be set directly; use a checkpoint resume");
+            case CLIP_FIELD_F_POST:
+            case CLIP_FIELD_G_POST: {
+                const bool is_f = field == CLIP_FIELD_F_POST;
+                std::vector<double> a0 = is_f ? direct_f_interior : direct_g_interior;
+                replace_interior(a0, in);
+                upload_pop(is_f ? fbuf[0] : gbuf[0], a0, nullptr);
+                break;
+            }
+            case CLIP_FIELD_F:
+            case CLIP_FIELD_G: {
+                const bool is_f = field == CLIP_FIELD_F;
+                std::vector<double> post;
+                get_field(is_f ? CLIP_FIELD_F_POST : CLIP_FIELD_G_POST, post);
+                std::vector<double> a0 = in;
+                replace_interior(a0, post);
+                upload_pop(is_f ? fbuf[0] : gbuf[0], a0, nullptr);
+                upload_pop(is_f ? fbuf[1] : gbuf[1], in, nullptr);
+                (is_f ? direct_f_interior : direct_g_interior) = in;
+                break;
+            }
         }
-        // Recompute the pending collision from the modified state.
-        if (initialized) recollide();
+        if (bc.is_velocity && (field == CLIP_FIELD_F || field == CLIP_FIELD_G || field == CLIP_FIELD_PHI)) {
+            inlet_fill(phib[phc]);
+            // keep the host copies of dev_f / dev_g in step with the ghosts
+            std::vector<double> raw;
+            for (int is_f = 0; is_f < 2; ++is_f) {
+                const int tb = 256;
+                const long long count = (long long)g.nodes * QL;
+                k_export_raw<L, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(
+                    is_f ? fbuf[1] : gbuf[1], scratch, g, bc, nullptr);
+                launch_check("k_export_raw");
+                download(scratch, (size_t)count, raw);
+                std::vector<double>& keep = is_f ? direct_f_interior : direct_g_interior;
+                for (int k = 0; k < g.ext[2]; ++k)
+                    for (int j = 0; j < g.ext[1]; ++j)
+                        for (int i = 0; i < g.ext[0]; ++i) {
+                            const int c[3] = {i, j, k};
+                            if (g.interior(c, L::D)) continue;
+                            for (int q = 0; q < QL; ++q) {
+                                const size_t t = (size_t)q * g.nodes + g.idx(i, j, k);
+                                keep[t] = raw[t];
+                            }
+                        }
+            }
+        }
+        recollide();
+    }
+
+    // NSAllen::macroscopic (NsAllen.cu:866-877) on the current f_post /
+    // g_post: phi = sum g_post at interior nodes (the ghost ring is left as
+    // it is), mu and grad phi from it, p = sum f_post and u from f_post with
+    // u_old = the current u; then the pending collision.
+    void macroscopic() override {
+        materialise();
+        HIP_CHECK(hipMemcpyAsync(phib[1 - phc], phib[phc], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice,
+                                 stream));
+        // pull buffer 0 in place, u[ucur] -> u[1-ucur], outflow planes -> prev[pcur]
+        StepArgs<ST, CT> a = args(0, ucur, 1 - pcur, phib[1 - phc], nullptr);
+        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+        launch_check("k_phase");
+        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+        launch_check("k_fused");
+        const int tb = 256;
+        k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(a, true, scratch, p);
+        launch_check("k_export_post");
+        HIP_CHECK(hipStreamSynchronize(stream));
+        phc = 1 - phc;
+        ucur = 1 - ucur;
+    }
+
+    // phi, the phi the pulled buffer is encoded against, u and p of a handle
+    // in the direct state (false otherwise).
+    bool direct_state(std::vector<double>& phi_now, std::vector<double>& phi_ref, std::vector<double>& uu,
+                      std::vector<double>& pp) override {
+        if (!direct_old) return false;
+        download(phib[phc], (size_t)g.nodes, phi_now);
+        download(phib[1 - phc], (size_t)g.nodes, phi_ref);
+        download(u[ucur], (size_t)g.nodes * L::D, uu);
+        download(p, (size_t)g.nodes, pp);
+        return true;
+    }
+
+    // Restart into the direct state (the counterpart of direct_state, with
+    // dev_f, dev_g, f_post, g_post from the checkpoint files); `raw_f` /
+    // `raw_g` are the stored 16-bit bytes of the f_post / g_post buffer,
+    // encoded against phi_ref unless `ref_zero`.
+    void resume_direct(const std::vector<double>& f, const std::vector<double>& gg, const std::vector<double>& fpost,
+                       const std::vector<double>& gpost, const std::vector<double>& phi_now,
+                       const std::vector<double>& phi_ref, const std::vector<double>& uu,
+                       const std::vector<double>& pp, bool ref_zero, const std::vector<unsigned char>* raw_f,
+                       const std::vector<unsigned char>* raw_g) override {
+        upload(phib[0], phi_ref);
+        upload(phib[1], phi_now);
+        phc = 1;
+        ucur = 1;
+        pcur = 0;
+        upload(u[1], uu);
+        upload(p, pp);
+        set_direct(fpost, gpost, f, gg);
+        const size_t raw_bytes = (size_t)g.nodes * QL * sizeof(ST);
+        if (sizeof(ST) == 2 && raw_f && raw_g && raw_f->size() == raw_bytes && raw_g->size() == raw_bytes) {
+            HIP_CHECK(hipMemcpyAsync(fbuf[0], raw_f->data(), raw_bytes, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(gbuf[0], raw_g->data(), raw_bytes, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            old_ref_zero = ref_zero;
+        }
+        initialized = true;
+        recollide();
     }
 
     // Re-runs the collision that produced pop[cur] from the state the last
@@ -659,7 +837,7 @@ struct SolverImpl : Impl {
 
     bool raw_pulled(bool is_f, std::vector<unsigned char>& out) override {
         if constexpr (sizeof(ST) != 2) return false;
-        if (direct_old || !initialized) return false;
+        if (!initialized) return false;
         const size_t bytes = (size_t)g.nodes * QL * sizeof(ST);
         out.resize(bytes);
         HIP_CHECK(hipMemcpyAsync(out.data(), is_f ? fbuf[1 - cur] : gbuf[1 - cur], bytes, hipMemcpyDeviceToHost,
@@ -795,6 +973,7 @@ struct SolverImpl : Impl {
             s_.fout = (ST*)st_f(b);
             s_.gout = (ST*)st_g(b);
             s_.uin = st_u(a);
+            s_.ufc = st_u(a);
             s_.ucol = st_u(a);
             s_.uout = st_u(b);
             ghost_refresh(st_phi(a), st_phi(b));

@@ -130,6 +130,11 @@ typedef struct clip_config {
     int inlet_modes;
     double inlet_mode_amp[CLIP_MAX_INLET_MODES];
     double inlet_mode_phase[CLIP_MAX_INLET_MODES];
+    /* optional mobility override (no reference counterpart): the RTI case
+       derives the mobility from Pe (InputData.cu:76-114); with the flag set
+       clip_config_derive keeps mobility_override instead, for every case */
+    int has_mobility_override;
+    double mobility_override;
 } clip_config;
 
 /* Sizes of a created solver. */
@@ -156,6 +161,11 @@ int clip_config_from_string(const char* text, int dim_override, clip_config* out
 /* Recompute derived parameters (InputData.cu:76-114) and validate. */
 int clip_config_derive(clip_config* cfg);
 
+/* Geometry::sdf (include/geometry/Geometry.cuh:102-161): signed distance
+ * (positive outside) from (x, y, z) to the first geometry object of `cfg`
+ * carrying `id`; 1e10 when none does.  Host only. */
+int clip_geometry_sdf(const clip_config* cfg, int id, double x, double y, double z, double* out);
+
 /* ---- solver lifecycle --------------------------------------------------- */
 /* Replaces the object graph built in main.cu:61-65 (Domain, DataArray,
  * NSAllen): allocates every device buffer for the configuration. */
@@ -174,9 +184,26 @@ int clip_synchronize(clip_solver* s);
  * host_out holds ndof*nodes doubles).  Replaces DataArray::updateHost
  * (DataArray.cu:131-138). */
 int clip_get_field(clip_solver* s, int field, int layout, double* host_out);
-/* Overwrite PHI, U, P, F, G, F_POST or G_POST from host memory (same shape as
- * clip_get_field).  The pending collision is recomputed afterwards. */
+/* Overwrite one of the reference's device arrays from host memory (same shape
+ * as clip_get_field): PHI (dev_c, ghost ring included), U (dev_vel), P
+ * (dev_p), F / G (dev_f / dev_g), F_POST / G_POST (dev_f_post / dev_g_post).
+ * The arrays keep their roles in the reference's solve(): the next collision
+ * reads f_post, g_post, p, u and phi (mu, grad phi and the normal are
+ * recomputed from the phi given) and overwrites the interior of dev_f /
+ * dev_g, whose ghost cells later pulls read.  Nothing is recomputed from the
+ * populations: call clip_macroscopic for that.  This is how
+ * CheckPointer::load (src/io/CheckPointer.cu:84-102) fills dev_f, dev_g,
+ * dev_f_post and dev_g_post before NSAllen::macroscopic (main.cu:67-71).
+ * RHO, MU, DPHI and NORMAL are derived and rejected (CLIP_ERR_ARG) before
+ * host_in is read. */
 int clip_set_field(clip_solver* s, int field, int layout, const double* host_in);
+/* NSAllen::macroscopic (NsAllen.cu:866-877, include/equation/NsAllen.cuh:89):
+ * phi = sum g_post at interior nodes (ghost ring left as it is), mu and
+ * grad phi from it, p = sum f_post, u from f_post with u_old = the current u
+ * (kernelMacroscopicf, NsAllen.cu:369-511); then the pending collision.
+ * clip_checkpoint_load already leaves the handle as CheckPointer::load +
+ * macroscopic() do (main.cu:67-71). */
+int clip_macroscopic(clip_solver* s);
 /* FP64 sum of a scalar field over interior nodes: Reporter::sum
  * (Reporter.cu:95-114).  field = PHI or RHO. */
 int clip_sum(clip_solver* s, int field, double* out);

