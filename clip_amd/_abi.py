@@ -112,6 +112,7 @@ SIGNATURES = {
     "clip_geometry_sdf": (ctypes.c_int, [ctypes.POINTER(ClipConfig), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _D]),
     "clip_sum": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "clip_sum_noalloc": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint), _P, _P, _P]),
     "clip_check_health": (ctypes.c_int, [_P, _D]),
     "clip_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "clip_checkpoint_load": (ctypes.c_int, [_P, ctypes.c_char_p]),

@@ -17,7 +17,7 @@ DRIVER = os.path.join(HERE, "clip")
 
 # (source, extra flags): the FP64 instantiation is built without FP contraction
 HIP_SOURCES = [("solver.hip", []), ("inst_f64.hip", ["-ffp-contract=off"]), ("inst_f32.hip", []),
-               ("inst_mixed.hip", [])]
+               ("inst_mixed.hip", []), ("noalloc.hip", [])]
 CPP_SOURCES = ["config.cpp"]
 ARCH = os.environ.get("CLIP_AMD_ARCH", "gfx950")
 

@@ -207,6 +207,22 @@ int clip_macroscopic(clip_solver* s);
 /* FP64 sum of a scalar field over interior nodes: Reporter::sum
  * (Reporter.cu:95-114).  field = PHI or RHO. */
 int clip_sum(clip_solver* s, int field, double* out);
+/* Allocation-free variant of clip_sum.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  FP64 sum
+ * (same kernels and order as clip_sum) of a device-resident scalar field over
+ * the interior nodes of a grid with CLIP's one-node ghost ring:
+ * `field` holds ext[0]*ext[1]*ext[2] values, x fastest (double for
+ * CLIP_PREC_F64, float otherwise; a handle's PHI in CLIP_LAYOUT_SOA order),
+ * dim 2 (ext[2] = 1) or 3, each used extent in [3, 2^20] (CLIP_ERR_ARG
+ * otherwise).  Writes the sum to `out` and, if non-NULL, the number of
+ * non-finite values to `nonfinite`, both device pointers, asynchronously on
+ * `hip_stream` (NULL = the null stream).  It allocates nothing: the reduction
+ * partials live in a fixed 8 KiB device array that each GPU keeps until the
+ * process ends and that every call on that GPU shares, so calls that overlap
+ * on one GPU race. */
+int clip_sum_noalloc(const void* field, int precision, int dim, const unsigned ext[3], double* out,
+                     unsigned* nonfinite, void* hip_stream);
 /* Health check of Reporter::print (Reporter.cu:124-144): sums PHI and returns
  * CLIP_ERR_DIVERGED (instead of throwing) when phi holds a NaN or Inf. */
 int clip_check_health(clip_solver* s, double* phi_sum);
