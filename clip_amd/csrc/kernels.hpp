@@ -45,12 +45,24 @@ enum : int {
     BC_DO_NOTHING = CLIP_BC_DO_NOTHING
 };
 
+// Device layout of a node field: node (i, j, k) at i + j*sy + k*sz.  The row
+// stride sy may exceed ext[0] (padding cells x >= ext[0] belong to no node),
+// so that with the buffer offsets of the solver every row's first interior
+// node starts a 128-byte line; nodes = sz * ext[2] is the per-field stride.
 struct DevGrid {
     int ext[3];
     int dmin[3], dmax[3];
     long long sy, sz;  // strides of j and k (i has stride 1)
     long long nodes;
     __host__ __device__ long long idx(int i, int j, int k) const { return i + j * sy + k * sz; }
+    // Coordinates of device index n; false for a padding cell.
+    __host__ __device__ bool coords(long long n, int c[3]) const {
+        c[2] = (int)(n / sz);
+        const long long r = n - (long long)c[2] * sz;
+        c[1] = (int)(r / sy);
+        c[0] = (int)(r - (long long)c[1] * sy);
+        return c[0] < ext[0];
+    }
     __host__ __device__ bool interior(const int c[3], int D) const {
         bool in = c[0] >= dmin[0] && c[0] <= dmax[0] && c[1] >= dmin[1] && c[1] <= dmax[1];
         if (D == 3) in = in && c[2] >= dmin[2] && c[2] <= dmax[2];
@@ -1200,9 +1212,7 @@ __global__ void k_init(StepArgs<ST, CT> a, ST* f2, ST* g2, CT L2, CT sigma) {
     const DevGrid& g = a.g;
     if (n >= g.nodes) return;
     int c[3];
-    c[0] = (int)(n % g.ext[0]);
-    c[1] = (int)((n / g.ext[0]) % g.ext[1]);
-    c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
+    if (!g.coords(n, c)) return;
     const Phys<CT>& ph = a.ph;
     const CT c0 = a.phi[n];
     const CT rho = ph.rho_l + c0 * (ph.rho_h - ph.rho_l);
@@ -1321,9 +1331,7 @@ __global__ void k_export_post(StepArgs<ST, CT> a, bool is_f, CT* out, CT* pout) 
     const DevGrid& g = a.g;
     if (n >= g.nodes) return;
     int c[3];
-    c[0] = (int)(n % g.ext[0]);
-    c[1] = (int)((n / g.ext[0]) % g.ext[1]);
-    c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
+    if (!g.coords(n, c)) return;
     const bool interior = g.interior(c, L::D);
     const CT* ref = is_f ? nullptr : a.phi_ref;
     CT sum = 0;
@@ -1347,8 +1355,8 @@ __global__ void k_export_raw(const ST* src, CT* out, DevGrid g, DevBC bc, const 
     if (t >= g.nodes * L::Q) return;
     const long long n = t % g.nodes;
     const int q = (int)(t / g.nodes);
-    const int c[3] = {(int)(n % g.ext[0]), (int)((n / g.ext[0]) % g.ext[1]),
-                      (int)(n / ((long long)g.ext[0] * g.ext[1]))};
+    int c[3];
+    if (!g.coords(n, c)) return;
     out[t] = SP::ld(src, t, q, g.interior(c, L::D) ? gref<SP>(ref, ref_index<L>(g, bc, c, q)) : CT(0));
 }
 
@@ -1358,8 +1366,8 @@ __global__ void k_import_raw(ST* dst, const CT* in, DevGrid g, DevBC bc, const C
     if (t >= g.nodes * L::Q) return;
     const long long n = t % g.nodes;
     const int q = (int)(t / g.nodes);
-    const int c[3] = {(int)(n % g.ext[0]), (int)((n / g.ext[0]) % g.ext[1]),
-                      (int)(n / ((long long)g.ext[0] * g.ext[1]))};
+    int c[3];
+    if (!g.coords(n, c)) return;
     SP::st(dst, t, q, in[t], g.interior(c, L::D) ? gref<SP>(ref, ref_index<L>(g, bc, c, q)) : CT(0));
 }
 
@@ -1369,9 +1377,7 @@ __global__ void k_export_stencil(const CT* phi, DevGrid g, Phys<CT> ph, CT* mu, 
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodes) return;
     int c[3];
-    c[0] = (int)(n % g.ext[0]);
-    c[1] = (int)((n / g.ext[0]) % g.ext[1]);
-    c[2] = (int)(n / ((long long)g.ext[0] * g.ext[1]));
+    if (!g.coords(n, c)) return;
     if (!g.interior(c, L::D)) {
         mu[n] = 0;
         for (int d = 0; d < L::D; ++d) dphi[d * g.nodes + n] = normal[d * g.nodes + n] = 0;

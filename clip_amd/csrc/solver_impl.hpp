@@ -64,12 +64,21 @@ struct SolverImpl : Impl {
     int cur = 1, ucur = 0, pcur = 0, phc = 0;
     unsigned tbx = BX, tby = BY;  // block shape of the bulk launches (tbx * tby = BX * BY)
     bool timing_on = false;
+    bool row_align = true;
     hipEvent_t ev[8] = {};
     double t_total = 0, t_ghost = 0, t_phase = 0, t_fused = 0;
     std::vector<void*> allocations;
 
+    // Node array of `count` elements (a multiple of the padded node count),
+    // offset so that element dmin = 1 of every row starts a 128-byte line.
     template <class T>
-    T* alloc(size_t count) {
+    T* alloc_grid(size_t count) {
+        const size_t lead = row_align ? 128 / sizeof(T) - 1 : 0;
+        return alloc<T>(count + lead + 1, lead);
+    }
+
+    template <class T>
+    T* alloc(size_t count, size_t lead = 0) {
         void* ptr = nullptr;
         if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
             (void)hipGetLastError();
@@ -77,7 +86,34 @@ struct SolverImpl : Impl {
         }
         allocations.push_back(ptr);
         bytes += count * sizeof(T);
-        return (T*)ptr;
+        return (T*)ptr + lead;
+    }
+
+    // Host arrays crossing the Impl interface are dense (x fastest over
+    // ext[0], ghosts included); device node arrays use the padded stride.
+    size_t hn() const { return (size_t)g.ext[0] * g.ext[1] * g.ext[2]; }
+    size_t hidx(int i, int j, int k) const { return (size_t)i + (size_t)g.ext[0] * ((size_t)j + (size_t)g.ext[1] * k); }
+    // Visits every node row: fn(dense offset of (0, j, k), device offset of (0, j, k)).
+    template <class Fn>
+    void for_rows(Fn fn) const {
+        for (int k = 0; k < g.ext[2]; ++k)
+            for (int j = 0; j < g.ext[1]; ++j) fn(hidx(0, j, k), (size_t)g.idx(0, j, k));
+    }
+    template <class V, class W>
+    void dense_to_dev(const V& in, W& out, int ndof) const {
+        out.assign((size_t)g.nodes * ndof, 0);
+        for (int d = 0; d < ndof; ++d)
+            for_rows([&](size_t h, size_t dv) {
+                for (int i = 0; i < g.ext[0]; ++i) out[(size_t)d * g.nodes + dv + i] = in[(size_t)d * hn() + h + i];
+            });
+    }
+    template <class V, class W>
+    void dev_to_dense(const V& in, W& out, int ndof) const {
+        out.assign(hn() * ndof, 0);
+        for (int d = 0; d < ndof; ++d)
+            for_rows([&](size_t h, size_t dv) {
+                for (int i = 0; i < g.ext[0]; ++i) out[(size_t)d * hn() + h + i] = in[(size_t)d * g.nodes + dv + i];
+            });
     }
 
     explicit SolverImpl(const clip_config& c) : cfg(c) {
@@ -108,8 +144,19 @@ struct SolverImpl : Impl {
             tbx = (unsigned)((per + 63) / 64 * 64);
             tby = std::max(1u, (unsigned)(BX * BY) / tbx);
         }
-        g.sy = g.ext[0];
-        g.sz = (long long)g.ext[0] * g.ext[1];
+        // Row stride padded so that every row's first interior node starts a
+        // 128-byte line in every node array (with the offsets of alloc_grid):
+        // a row's collision output then covers whole lines, which no other
+        // block (on another XCD's L2) shares.  Measured on a 19-plane 16-bit
+        // pull kernel on the 256^3 grid: 3.76 -> 4.24 TB/s.
+        // CLIP_AMD_ROW_ALIGN=0 keeps the dense stride (ext_x).
+        {
+            const char* env = std::getenv("CLIP_AMD_ROW_ALIGN");
+            row_align = !(env && env[0] == '0');
+            const long long unit = 128 / (long long)std::min(sizeof(ST), sizeof(CT));
+            g.sy = row_align ? (g.ext[0] + unit - 1) / unit * unit : g.ext[0];
+        }
+        g.sz = g.sy * g.ext[1];
         g.nodes = g.sz * g.ext[2];
         for (int s = 0; s < 6; ++s) bc.type[s] = (s < 2 * L::D) ? c.bc_type[s] : -1;
         bool isWall = false, isFC = false, isNeu = false, isVel = false, isSlip = false;
@@ -147,13 +194,13 @@ struct SolverImpl : Impl {
         for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
         const size_t nodes = (size_t)g.nodes;
         for (int b = 0; b < 2; ++b) {
-            fbuf[b] = alloc<ST>(nodes * QL);
-            gbuf[b] = alloc<ST>(nodes * QL);
-            u[b] = alloc<CT>(nodes * L::D);
-            phib[b] = alloc<CT>(nodes);
+            fbuf[b] = alloc_grid<ST>(nodes * QL);
+            gbuf[b] = alloc_grid<ST>(nodes * QL);
+            u[b] = alloc_grid<CT>(nodes * L::D);
+            phib[b] = alloc_grid<CT>(nodes);
         }
-        p = alloc<CT>(nodes);
-        scratch = alloc<CT>(nodes * std::max(QL, L::D));
+        p = alloc_grid<CT>(nodes);
+        scratch = alloc_grid<CT>(nodes * std::max(QL, L::D));
         if (shell_nodes() > 0) {
             if (shell_nodes() * QL >= (1LL << 31)) throw ConfigError("boundary layer too large");
             shell_f = alloc<CT>((size_t)shell_nodes() * QL);
@@ -488,6 +535,37 @@ struct SolverImpl : Impl {
         for (size_t i = 0; i < count; ++i) out[i] = (double)h[i];
     }
 
+    // Node field of `ndof` components: device layout <-> dense host array.
+    template <class T>
+    void download_field(const T* dev, int ndof, std::vector<double>& out) {
+        std::vector<double> raw;
+        download(dev, (size_t)g.nodes * ndof, raw);
+        dev_to_dense(raw, out, ndof);
+    }
+    template <class T>
+    void upload_field(T* dev, const std::vector<double>& in, int ndof) {
+        std::vector<double> d;
+        dense_to_dev(in, d, ndof);
+        upload(dev, d);
+    }
+    // Stored bytes of a population buffer: device layout <-> dense.
+    void raw_to_dense(const std::vector<unsigned char>& in, std::vector<unsigned char>& out) const {
+        out.assign(hn() * QL * sizeof(ST), 0);
+        for (int q = 0; q < QL; ++q)
+            for_rows([&](size_t h, size_t dv) {
+                std::memcpy(&out[((size_t)q * hn() + h) * sizeof(ST)], &in[((size_t)q * g.nodes + dv) * sizeof(ST)],
+                            (size_t)g.ext[0] * sizeof(ST));
+            });
+    }
+    void raw_to_dev(const std::vector<unsigned char>& in, std::vector<unsigned char>& out) const {
+        out.assign((size_t)g.nodes * QL * sizeof(ST), 0);
+        for (int q = 0; q < QL; ++q)
+            for_rows([&](size_t h, size_t dv) {
+                std::memcpy(&out[((size_t)q * g.nodes + dv) * sizeof(ST)], &in[((size_t)q * hn() + h) * sizeof(ST)],
+                            (size_t)g.ext[0] * sizeof(ST));
+            });
+    }
+
     // phi the pulled buffer pop[1-cur] is encoded with (nullptr = 0).
     const CT* old_ref() const { return old_ref_zero ? nullptr : phib[1 - phc]; }
 
@@ -511,16 +589,16 @@ struct SolverImpl : Impl {
         const int tb = 256;
         CT* phi = phib[phc];
         switch (field) {
-            case CLIP_FIELD_PHI: download(phi, nodes, out); return;
+            case CLIP_FIELD_PHI: download_field(phi, 1, out); return;
             case CLIP_FIELD_RHO: {
-                download(phi, nodes, out);
+                download_field(phi, 1, out);
                 for (auto& v : out) v = (double)(ph.rho_l + (CT)v * (ph.rho_h - ph.rho_l));
                 return;
             }
-            case CLIP_FIELD_U: download(u[ucur], nodes * L::D, out); return;
+            case CLIP_FIELD_U: download_field(u[ucur], L::D, out); return;
             case CLIP_FIELD_P: {
                 if (!direct_old) export_post(true);
-                download(p, nodes, out);
+                download_field(p, 1, out);
                 return;
             }
             case CLIP_FIELD_MU:
@@ -531,9 +609,9 @@ struct SolverImpl : Impl {
                 CT* nr = scratch + nodes * (1 + L::D);
                 k_export_stencil<L, CT><<<(unsigned)((nodes + tb - 1) / tb), tb, 0, stream>>>(phi, g, ph, mu, dp, nr);
                 launch_check("k_export_stencil");
-                if (field == CLIP_FIELD_MU) download(mu, nodes, out);
-                else if (field == CLIP_FIELD_DPHI) download(dp, nodes * L::D, out);
-                else download(nr, nodes * L::D, out);
+                if (field == CLIP_FIELD_MU) download_field(mu, 1, out);
+                else if (field == CLIP_FIELD_DPHI) download_field(dp, L::D, out);
+                else download_field(nr, L::D, out);
                 return;
             }
             case CLIP_FIELD_F:
@@ -545,7 +623,7 @@ struct SolverImpl : Impl {
                 k_export_raw<L, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(
                     src, scratch, g, bc, is_f ? nullptr : old_ref());
                 launch_check("k_export_raw");
-                download(scratch, count, out);
+                download_field(scratch, QL, out);
                 const auto& keep = is_f ? direct_f_interior : direct_g_interior;
                 if (direct_old && !keep.empty()) out = keep;
                 if (!direct_old) fill_periodic_ghosts(out, QL);
@@ -554,7 +632,7 @@ struct SolverImpl : Impl {
             case CLIP_FIELD_F_POST:
             case CLIP_FIELD_G_POST: {
                 export_post(field == CLIP_FIELD_F_POST);
-                download(scratch, nodes * QL, out);
+                download_field(scratch, QL, out);
                 return;
             }
         }
@@ -574,7 +652,7 @@ struct SolverImpl : Impl {
                         int s[3] = {i, j, k};
                         s[ax] = (c[ax] == 0) ? g.dmax[ax] : g.dmin[ax];
                         for (int q = 0; q < ndof; ++q)
-                            a[(size_t)q * g.nodes + g.idx(i, j, k)] = a[(size_t)q * g.nodes + g.idx(s[0], s[1], s[2])];
+                            a[(size_t)q * hn() + hidx(i, j, k)] = a[(size_t)q * hn() + hidx(s[0], s[1], s[2])];
                     }
         }
     }
@@ -590,8 +668,8 @@ struct SolverImpl : Impl {
     // Encodes populations (dense, SoA) into a population buffer; `ref` = phi
     // at interior nodes for g (nullptr: absolute).
     void upload_pop(ST* dev, const std::vector<double>& in, const CT* ref) {
-        const long long count = (long long)in.size();
-        upload(scratch, in);
+        const long long count = (long long)g.nodes * QL;
+        upload_field(scratch, in, QL);
         const int tb = 256;
         k_import_raw<L, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(dev, scratch, g, bc, ref);
         launch_check("k_import_raw");
@@ -605,7 +683,7 @@ struct SolverImpl : Impl {
             for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
                 for (int i = g.dmin[0]; i <= g.dmax[0]; ++i)
                     for (int q = 0; q < QL; ++q) {
-                        const size_t t = (size_t)q * g.nodes + g.idx(i, j, k);
+                        const size_t t = (size_t)q * hn() + hidx(i, j, k);
                         dst[t] = src[t];
                     }
     }
@@ -662,12 +740,12 @@ struct SolverImpl : Impl {
                 // inlet disc, sides without a mirror) are read from whichever
                 // buffer is current.  The direct state stores f_post without a
                 // phi reference, so the other buffer is free.
-                upload(phib[phc], in);
+                upload_field(phib[phc], in, 1);
                 HIP_CHECK(hipMemcpyAsync(phib[1 - phc], phib[phc], (size_t)g.nodes * sizeof(CT),
                                          hipMemcpyDeviceToDevice, stream));
                 break;
-            case CLIP_FIELD_U: upload(u[ucur], in); break;
-            case CLIP_FIELD_P: upload(p, in); break;
+            case CLIP_FIELD_U: upload_field(u[ucur], in, L::D); break;
+            case CLIP_FIELD_P: upload_field(p, in, 1); break;
             case CLIP_FIELD_F_POST:
             case CLIP_FIELD_G_POST: {
                 const bool is_f = field == CLIP_FIELD_F_POST;
@@ -699,7 +777,7 @@ struct SolverImpl : Impl {
                 k_export_raw<L, ST, CT, SP><<<(unsigned)((count + tb - 1) / tb), tb, 0, stream>>>(
                     is_f ? fbuf[1] : gbuf[1], scratch, g, bc, nullptr);
                 launch_check("k_export_raw");
-                download(scratch, (size_t)count, raw);
+                download_field(scratch, QL, raw);
                 std::vector<double>& keep = is_f ? direct_f_interior : direct_g_interior;
                 for (int k = 0; k < g.ext[2]; ++k)
                     for (int j = 0; j < g.ext[1]; ++j)
@@ -707,7 +785,7 @@ struct SolverImpl : Impl {
                             const int c[3] = {i, j, k};
                             if (g.interior(c, L::D)) continue;
                             for (int q = 0; q < QL; ++q) {
-                                const size_t t = (size_t)q * g.nodes + g.idx(i, j, k);
+                                const size_t t = (size_t)q * hn() + hidx(i, j, k);
                                 keep[t] = raw[t];
                             }
                         }
@@ -743,10 +821,10 @@ struct SolverImpl : Impl {
     bool direct_state(std::vector<double>& phi_now, std::vector<double>& phi_ref, std::vector<double>& uu,
                       std::vector<double>& pp) override {
         if (!direct_old) return false;
-        download(phib[phc], (size_t)g.nodes, phi_now);
-        download(phib[1 - phc], (size_t)g.nodes, phi_ref);
-        download(u[ucur], (size_t)g.nodes * L::D, uu);
-        download(p, (size_t)g.nodes, pp);
+        download_field(phib[phc], 1, phi_now);
+        download_field(phib[1 - phc], 1, phi_ref);
+        download_field(u[ucur], L::D, uu);
+        download_field(p, 1, pp);
         return true;
     }
 
@@ -759,18 +837,21 @@ struct SolverImpl : Impl {
                        const std::vector<double>& phi_ref, const std::vector<double>& uu,
                        const std::vector<double>& pp, bool ref_zero, const std::vector<unsigned char>* raw_f,
                        const std::vector<unsigned char>* raw_g) override {
-        upload(phib[0], phi_ref);
-        upload(phib[1], phi_now);
+        upload_field(phib[0], phi_ref, 1);
+        upload_field(phib[1], phi_now, 1);
         phc = 1;
         ucur = 1;
         pcur = 0;
-        upload(u[1], uu);
-        upload(p, pp);
+        upload_field(u[1], uu, L::D);
+        upload_field(p, pp, 1);
         set_direct(fpost, gpost, f, gg);
-        const size_t raw_bytes = (size_t)g.nodes * QL * sizeof(ST);
+        const size_t raw_bytes = hn() * QL * sizeof(ST);
         if (sizeof(ST) == 2 && raw_f && raw_g && raw_f->size() == raw_bytes && raw_g->size() == raw_bytes) {
-            HIP_CHECK(hipMemcpyAsync(fbuf[0], raw_f->data(), raw_bytes, hipMemcpyHostToDevice, stream));
-            HIP_CHECK(hipMemcpyAsync(gbuf[0], raw_g->data(), raw_bytes, hipMemcpyHostToDevice, stream));
+            std::vector<unsigned char> df, dg;
+            raw_to_dev(*raw_f, df);
+            raw_to_dev(*raw_g, dg);
+            HIP_CHECK(hipMemcpyAsync(fbuf[0], df.data(), df.size(), hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(gbuf[0], dg.data(), dg.size(), hipMemcpyHostToDevice, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
             old_ref_zero = ref_zero;
         }
@@ -811,7 +892,7 @@ struct SolverImpl : Impl {
                         if (c[a] != ((s & 1) ? g.dmax[a] : g.dmin[a])) continue;
                         for (int q = 0; q < QL; ++q)
                             if (L::e(q, a) == inward)
-                                fn(s, q, g.idx(i, j, k), (long long)q * plane_size(g, s) + plane_index(g, s, c));
+                                fn(s, q, (long long)hidx(i, j, k), (long long)q * plane_size(g, s) + plane_index(g, s, c));
                     }
         }
     }
@@ -825,24 +906,25 @@ struct SolverImpl : Impl {
             std::vector<double> plane;
             download(is_f ? prevf[set][s] : prevg[set][s], (size_t)plane_size(g, s) * QL, plane);
             for_outflow_entries([&](int side, int q, long long node, long long off) {
-                if (side == s) out[(size_t)q * g.nodes + node] = plane[off];
+                if (side == s) out[(size_t)q * hn() + node] = plane[off];
             });
         }
     }
 
     void phi_pair(std::vector<double>& now, std::vector<double>& before) override {
-        download(phib[phc], (size_t)g.nodes, now);
-        download(phib[1 - phc], (size_t)g.nodes, before);
+        download_field(phib[phc], 1, now);
+        download_field(phib[1 - phc], 1, before);
     }
 
     bool raw_pulled(bool is_f, std::vector<unsigned char>& out) override {
         if constexpr (sizeof(ST) != 2) return false;
         if (!initialized) return false;
         const size_t bytes = (size_t)g.nodes * QL * sizeof(ST);
-        out.resize(bytes);
-        HIP_CHECK(hipMemcpyAsync(out.data(), is_f ? fbuf[1 - cur] : gbuf[1 - cur], bytes, hipMemcpyDeviceToHost,
+        std::vector<unsigned char> dev(bytes);
+        HIP_CHECK(hipMemcpyAsync(dev.data(), is_f ? fbuf[1 - cur] : gbuf[1 - cur], bytes, hipMemcpyDeviceToHost,
                                  stream));
         HIP_CHECK(hipStreamSynchronize(stream));
+        raw_to_dense(dev, out);
         return true;
     }
 
@@ -857,14 +939,17 @@ struct SolverImpl : Impl {
             // the phi that wrote them, outflow planes from dev_f_prev /
             // dev_g_prev, u_old and both phi fields from the extension; then the
             // step kernels recompute macroscopic() and the pending collision.
-            upload(phib[0], *phi_before);
-            upload(phib[1], *phi_now);
-            const size_t raw_bytes = (size_t)g.nodes * QL * sizeof(ST);
+            upload_field(phib[0], *phi_before, 1);
+            upload_field(phib[1], *phi_now, 1);
+            const size_t raw_bytes = hn() * QL * sizeof(ST);
             if (sizeof(ST) == 2 && raw_f && raw_g && raw_f->size() == raw_bytes && raw_g->size() == raw_bytes) {
                 // the stored 16-bit bytes themselves: no re-encoding
+                std::vector<unsigned char> df, dg;
+                raw_to_dev(*raw_f, df);
+                raw_to_dev(*raw_g, dg);
                 for (int b = 0; b < 2; ++b) {
-                    HIP_CHECK(hipMemcpyAsync(fbuf[b], raw_f->data(), raw_bytes, hipMemcpyHostToDevice, stream));
-                    HIP_CHECK(hipMemcpyAsync(gbuf[b], raw_g->data(), raw_bytes, hipMemcpyHostToDevice, stream));
+                    HIP_CHECK(hipMemcpyAsync(fbuf[b], df.data(), df.size(), hipMemcpyHostToDevice, stream));
+                    HIP_CHECK(hipMemcpyAsync(gbuf[b], dg.data(), dg.size(), hipMemcpyHostToDevice, stream));
                 }
                 HIP_CHECK(hipStreamSynchronize(stream));
             } else {
@@ -873,14 +958,14 @@ struct SolverImpl : Impl {
                 upload_pop(fbuf[1], f, nullptr);
                 upload_pop(gbuf[1], gg, phib[0]);
             }
-            upload(u[0], *uold);
+            upload_field(u[0], *uold, L::D);
             for (int s = 0; s < 2 * L::D; ++s) {
                 if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;
                 std::vector<double> pf((size_t)plane_size(g, s) * QL, 0.0), pg(pf.size(), 0.0);
                 for_outflow_entries([&](int side, int q, long long node, long long off) {
                     if (side != s) return;
-                    pf[off] = fprev ? (*fprev)[(size_t)q * g.nodes + node] : fpost[(size_t)q * g.nodes + node];
-                    pg[off] = gprev ? (*gprev)[(size_t)q * g.nodes + node] : gpost[(size_t)q * g.nodes + node];
+                    pf[off] = fprev ? (*fprev)[(size_t)q * hn() + node] : fpost[(size_t)q * hn() + node];
+                    pg[off] = gprev ? (*gprev)[(size_t)q * hn() + node] : gpost[(size_t)q * hn() + node];
                 });
                 upload(prevf[0][s], pf);
                 upload(prevg[0][s], pg);
@@ -908,7 +993,7 @@ struct SolverImpl : Impl {
             for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
                 for (int i = g.dmin[0]; i <= g.dmax[0]; ++i)
                     for (int q = 0; q < QL; ++q) {
-                        const size_t t = (size_t)q * g.nodes + g.idx(i, j, k);
+                        const size_t t = (size_t)q * hn() + hidx(i, j, k);
                         a0[t] = fpost[t];
                         b0[t] = gpost[t];
                     }
@@ -916,8 +1001,8 @@ struct SolverImpl : Impl {
         upload_pop(gbuf[0], b0, nullptr);
         upload_pop(fbuf[1], f, nullptr);
         upload_pop(gbuf[1], gg, nullptr);
-        std::vector<double> zero((size_t)g.nodes * L::D, 0.0);
-        upload(u[0], zero);
+        std::vector<double> zero(hn() * L::D, 0.0);
+        upload_field(u[0], zero, L::D);
         StepArgs<ST, CT> a = args(0, 0, 1, phib[1], nullptr);
         k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
         launch_check("k_phase");
@@ -1200,7 +1285,7 @@ struct SolverImpl : Impl {
         adj_ready = false;
     }
 
-    void uold_soa(std::vector<double>& out) override { download(u[1 - ucur], (size_t)g.nodes * L::D, out); }
+    void uold_soa(std::vector<double>& out) override { download_field(u[1 - ucur], L::D, out); }
 
     double reduce(int field, unsigned* nonfinite) override {
         HIP_CHECK(hipMemsetAsync(red_bad, 0, sizeof(unsigned), stream));
