@@ -241,10 +241,13 @@ class Solver:
         _abi.check(self._lib.clip_enable_timing(self._h, 1 if on else 0))
 
     # ---- reverse mode (fully periodic domains, "f64" / "f32" handles) ----
-    def adjoint_forward(self, phi0, nsteps, checkpoint_every=50):
+    def adjoint_forward(self, phi0, nsteps, checkpoint_every=50, inner_every=0):
         """Runs nsteps steps from the interior phase field phi0 (array of the
         interior shape, index [i, j(, k)]) and returns phi after the last
-        step (same shape); keeps states for adjoint_backward."""
+        step (same shape); keeps states for adjoint_backward (every
+        checkpoint_every steps, and with 0 < inner_every < checkpoint_every a
+        second level inside each recomputed segment)."""
+        _abi.check(self._lib.clip_adjoint_set_nesting(self._h, int(inner_every)))
         shape = self.config.interior_shape()
         a = np.ascontiguousarray(np.asarray(phi0, dtype=np.float64).reshape(shape).transpose()).ravel()
         out = np.empty_like(a)
@@ -267,6 +270,15 @@ class Solver:
                                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                    ctypes.byref(gs), ctypes.byref(gm)))
         return out.reshape(shape[::-1]).transpose().copy(), gs.value, gm.value
+
+    def adjoint_inlet_gradient(self):
+        """dL/dA_m of the inlet modes (inletModeAmplitude) from the last
+        adjoint_backward (numpy array of length inlet_modes)."""
+        n = int(self.config.raw.inlet_modes)
+        out = np.zeros(max(n, 1))
+        _abi.check(self._lib.clip_adjoint_inlet_gradient(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                         n))
+        return out[:n]
 
     def last_timing(self):
         vals = [ctypes.c_double() for _ in range(4)]

@@ -123,6 +123,8 @@ SIGNATURES = {
     "clip_adjoint_forward": (ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int, _D]),
     "clip_adjoint_backward": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "clip_adjoint_set_normal_floor": (ctypes.c_int, [_P, ctypes.c_double]),
+    "clip_adjoint_set_nesting": (ctypes.c_int, [_P, ctypes.c_int]),
+    "clip_adjoint_inlet_gradient": (ctypes.c_int, [_P, _D, ctypes.c_int]),
     "clip_write_vts": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "clip_destroy": (None, [_P]),
     "clip_last_error": (ctypes.c_char_p, []),

@@ -1,18 +1,30 @@
-// adjoint.hpp — reverse-mode (adjoint) kernels of the coupled step on fully
-// periodic domains, FP64 or FP32 populations stored plainly.
+// adjoint.hpp — reverse-mode (adjoint) kernels of the coupled step, FP64 or
+// FP32 populations stored plainly, periodic or bounded by walls, velocity
+// inlets, free-convect outflows, Neumann and do-nothing sides.
 //
-// Forward step n (state S^n = f*, g*, u, phi with its ghost ring):
-//   f_post/g_post(x) = f*/g*(x - e_q)                (pull with periodic wrap)
-//   phi'(x)          = sum_q g_post(x)               (k_phase)
-//   stencil(x)       reads phi' at interior neighbours and phi^n (the lagged
-//                    ghost ring) where the neighbour is a ghost cell
+// Forward step n (state S^n = f*, g*, u, phi with its ghost ring, outflow
+// planes P = the previous step's f_post / g_post on free-convect sides):
+//   f_post/g_post(x) = f*/g*(x - e_q) in the bulk; the boundary chain
+//                    (Puller: inlet, bounce-back, free convect, Neumann) in the
+//                    boundary layer, reading S^n's populations, planes and u
+//   phi'(x)          = sum_q g_post(x) at interior nodes; the ghost ring of
+//                    phi' copies cells of phi^n (k_ghost_*: the lagged ring)
+//   stencil(x)       reads phi' at the 27 neighbours (ring cells included)
 //   node(x)          u, g*, f* from f_post, g_post, phi', grad, lap, u^n
-// The reverse sweep of one step runs three kernels:
-//   k_adj_node  node VJP (node.hpp) at every interior node
-//   k_adj_phi   transpose of the stencil: adjoint of phi' and of the lagged phi^n
-//   k_adj_pull  transpose of the pull: adjoint of f*, g* of S^n
+//   P^{n+1}          f_post / g_post on the outflow boundary nodes
+// The reverse sweep of one step runs:
+//   k_adj_node   node VJP (node.hpp) at every interior node (+ the adjoint
+//                of P^{n+1} on outflow boundary nodes)
+//   k_adj_phi    transpose of the stencil and of the phase sum's consumer:
+//                adjoint of phi' at every cell (interior and ring)
+//   k_adj_ring   transpose of the ring copy: adjoint of phi^n (host-built map)
+//   k_adj_pull   transpose of the bulk pull (gather, a permutation)
+//   k_adj_csr_*  transposes of the boundary-layer chains (host-built lists):
+//                populations, outflow planes, inlet values, and u through the
+//                free-convect weights
 #pragma once
 
+#include "adjoint_maps.hpp"
 #include "kernels.hpp"
 #include "node.hpp"
 
@@ -42,9 +54,18 @@ __device__ __forceinline__ void stencil_weights(int dx, int dy, int dz, T wg[3],
     }
 }
 
+// Row lists (CSR) of a host-built transpose, in device memory.
+struct AdjRows {
+    const long long* target;
+    const int* start;
+    const AdjEntry* e;
+    int rows;
+};
+
 template <class CT>
 struct AdjArgs {
     DevGrid g;
+    DevBC bc;
     Phys<CT> ph;
     const CT* F;     // f* of S^n (StorePlain), Q x nodes
     const CT* G;     // g* of S^n
@@ -61,6 +82,16 @@ struct AdjArgs {
     double* part;    // 4 x blocks partial sums: beta, kappa, mobility, sigma
     CT sigma, half_length;
     CT nfloor;       // regularisation of the normal's derivative (0 = exact)
+    // outflow planes (free-convect sides): S^n's planes, the adjoint of
+    // P^{n+1} (in) and of P^n (out); each a concatenation of the sides' plane
+    // sets at offsets poff[side]
+    const CT* pf;
+    const CT* pg;
+    const CT* lPf_in;
+    const CT* lPg_in;
+    CT* lPf_out;
+    CT* lPg_out;
+    long long poff[6];
 };
 
 __device__ __forceinline__ int wrap_axis(const DevGrid& g, int a, int v) {
@@ -68,6 +99,16 @@ __device__ __forceinline__ int wrap_axis(const DevGrid& g, int a, int v) {
     if (v < g.dmin[a]) return v + span;
     if (v > g.dmax[a]) return v - span;
     return v;
+}
+
+// Boundary layer: interior nodes next to a side that is not periodic on both
+// ends (their pulls resolve the boundary chain).
+template <int D>
+__device__ __forceinline__ bool in_shell(const DevGrid& g, const DevBC& bc, const int c[3]) {
+    bool s = false;
+#pragma unroll
+    for (int a = 0; a < D; ++a) s |= !bc.both_periodic[a] && (c[a] == g.dmin[a] || c[a] == g.dmax[a]);
+    return s;
 }
 
 template <int D>
@@ -111,14 +152,20 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_node(AdjArgs<CT> a) {
     double pb = 0, pk = 0, pm = 0;
     if (interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) {
         const long long n = g.idx(c[0], c[1], c[2]);
+        const bool shell = in_shell<D>(g, a.bc, c);
         CT fp[Q], gp[Q], lfs[Q], lgs[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int sx = wrap_axis(g, 0, c[0] - L::ex(q)), sy = wrap_axis(g, 1, c[1] - L::ey(q));
-            const int sz = (D == 3) ? wrap_axis(g, 2, c[2] - L::ez(q)) : c[2];
-            const long long src = g.idx(sx, sy, sz);
-            fp[q] = a.F[(long long)q * g.nodes + src];
-            gp[q] = a.G[(long long)q * g.nodes + src];
+            if (shell) {  // resolved by k_adj_shell_post into lfp / lgp
+                fp[q] = a.lfp[(long long)q * g.nodes + n];
+                gp[q] = a.lgp[(long long)q * g.nodes + n];
+            } else {
+                const int sx = wrap_axis(g, 0, c[0] - L::ex(q)), sy = wrap_axis(g, 1, c[1] - L::ey(q));
+                const int sz = (D == 3) ? wrap_axis(g, 2, c[2] - L::ez(q)) : c[2];
+                const long long src = g.idx(sx, sy, sz);
+                fp[q] = a.F[(long long)q * g.nodes + src];
+                gp[q] = a.G[(long long)q * g.nodes + src];
+            }
             lfs[q] = a.lF[(long long)q * g.nodes + n];
             lgs[q] = a.lG[(long long)q * g.nodes + n];
         }
@@ -132,6 +179,23 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_node(AdjArgs<CT> a) {
         }
         NodeAdj<L, CT> o{};
         node_reverse<L, CT>(a.ph, fp, gp, c0, dc, lap, uo, lfs, lgs, lu, o, a.nfloor);
+        // f_post / g_post of an outflow boundary node are also the next
+        // step's outflow planes (store_prev)
+        if (shell && a.bc.is_fc && a.lPf_in) {
+#pragma unroll
+            for (int side = 0; side < 2 * D; ++side) {
+                const int ax = side >> 1;
+                const int inward = (side & 1) ? -1 : 1;
+                if (a.bc.type[side] != BC_FREE_CONVECT || c[ax] != ((side & 1) ? g.dmax[ax] : g.dmin[ax])) continue;
+                const long long t = plane_index(g, side, c), ps = plane_size(g, side);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (L::e(q, ax) == inward) {
+                        o.lfp[q] += a.lPf_in[a.poff[side] + (long long)q * ps + t];
+                        o.lgp[q] += a.lPg_in[a.poff[side] + (long long)q * ps + t];
+                    }
+            }
+        }
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             a.lfp[(long long)q * g.nodes + n] = o.lfp[q];
@@ -151,37 +215,92 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_node(AdjArgs<CT> a) {
     block_sum4(pb, pk, pm, 0.0, a.part);
 }
 
-// Transpose of the stencil at interior node y.  LAG: the stencil of the step
-// read phi^n at ghost positions (adjoint goes to lPhi); otherwise (initial
-// state) every read is phi0 at the wrapped position.
-template <class L, class CT, bool LAG>
-__global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a) {
+// Forward f_post / g_post of the boundary-layer nodes (the Puller chain on
+// S^n) into lfp / lgp, which k_adj_node reads at those nodes and then
+// overwrites with the adjoints (same thread, same entries).  blockIdx.y is q,
+// so each wave resolves one compile-time direction (as k_shell_pull).
+template <class L, class CT, int Q0>
+__device__ __forceinline__ void adj_post_dir(const AdjArgs<CT>& a, const CT* const* pfs, const CT* const* pgs,
+                                             const int c[3], long long n) {
+    Puller<L, CT, CT, StorePlain<CT, CT>, false> Pf{a.F, a.g, a.bc, a.uo, pfs, nullptr};
+    Puller<L, CT, CT, StorePlain<CT, CT>, false> Pg{a.G, a.g, a.bc, a.uo, pgs, nullptr};
+    a.lfp[(long long)Q0 * a.g.nodes + n] = Pf.lvl3(c, Q0);
+    a.lgp[(long long)Q0 * a.g.nodes + n] = Pg.lvl3(c, Q0);
+}
+template <class L, class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_shell_post(AdjArgs<CT> a, ShellDesc sd) {
+    const unsigned S = (unsigned)shell_total(sd);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    int c[3];
+    if (t >= S || !shell_node_at<L>(a.g, sd, t, c)) return;
+    const long long n = a.g.idx(c[0], c[1], c[2]);
+    const CT* pfs[6];
+    const CT* pgs[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        pfs[s] = a.pf ? a.pf + a.poff[s] : nullptr;
+        pgs[s] = a.pg ? a.pg + a.poff[s] : nullptr;
+    }
+#define CLIP_ADJ_DIR(k)                                                  \
+    case k:                                                              \
+        if constexpr (k < L::Q) adj_post_dir<L, CT, k>(a, pfs, pgs, c, n); \
+        break;
+    switch (blockIdx.y) {
+        CLIP_ADJ_DIR(0) CLIP_ADJ_DIR(1) CLIP_ADJ_DIR(2) CLIP_ADJ_DIR(3) CLIP_ADJ_DIR(4) CLIP_ADJ_DIR(5)
+        CLIP_ADJ_DIR(6) CLIP_ADJ_DIR(7) CLIP_ADJ_DIR(8) CLIP_ADJ_DIR(9) CLIP_ADJ_DIR(10) CLIP_ADJ_DIR(11)
+        CLIP_ADJ_DIR(12) CLIP_ADJ_DIR(13) CLIP_ADJ_DIR(14) CLIP_ADJ_DIR(15) CLIP_ADJ_DIR(16) CLIP_ADJ_DIR(17)
+        CLIP_ADJ_DIR(18)
+        default: break;
+    }
+#undef CLIP_ADJ_DIR
+}
+
+// Transpose of the stencil at every cell y of phi' (interior and ghost ring):
+// lall[y] = lPhi[y] (adjoint from later steps) + the node's own phi term
+// (interior y) + the stencil weights of every interior node z within one
+// cell of y.
+template <class L, class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a, CT* lall) {
     constexpr int D = L::D;
     const DevGrid& g = a.g;
+    const long long y = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int c[3];
-    if (!interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-    const long long n = g.idx(c[0], c[1], c[2]);
-    CT total = a.lPhi[n] + a.lst[n];
-    CT lag = CT(0);
+    if (y >= g.nodes || !g.coords(y, c)) return;
+    CT total = a.lPhi[y];
+    if (g.interior(c, D)) total += a.lst[y];
     const int zlo = (D == 3) ? -1 : 0, zhi = (D == 3) ? 1 : 0;
     for (int dz = zlo; dz <= zhi; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
+                const int z[3] = {c[0] - dx, c[1] - dy, c[2] - dz};
+                if (!g.interior(z, D)) continue;
                 CT wg[3], wl;
                 stencil_weights<D, CT>(dx, dy, dz, wg, wl);
-                const int z[3] = {c[0] - dx, c[1] - dy, c[2] - dz};
-                const bool zin = g.interior(z, D);
-                const int zx = wrap_axis(g, 0, z[0]), zy = wrap_axis(g, 1, z[1]);
-                const int zz = (D == 3) ? wrap_axis(g, 2, z[2]) : z[2];
-                const long long m = g.idx(zx, zy, zz);
+                const long long m = g.idx(z[0], z[1], z[2]);
                 CT v = wl * a.lst[(long long)(1 + D) * g.nodes + m];
 #pragma unroll
                 for (int d = 0; d < D; ++d) v += wg[d] * a.lst[(long long)(1 + d) * g.nodes + m];
-                if (zin || !LAG) total += v;
-                else lag += v;
+                total += v;
             }
-    a.lphin[n] = total;
-    if (LAG) a.lPhi[n] = lag;
+    lall[y] = total;
+}
+
+// Transpose of a copy map (host-built rows: target = a cell of the source
+// field, entries' rd = the cells copied from it): out[target] = sum over the
+// row of in[rd].  Targets of no row are left as they are.
+template <class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_ring(AdjRows r, const CT* in, CT* out) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= r.rows) return;
+    CT s = CT(0);
+    for (int k = r.start[row]; k < r.start[row + 1]; ++k) s += in[r.e[k].rd];
+    out[r.target[row]] += s;
+}
+
+template <class CT>
+__global__ void k_adj_fill(CT* x, long long n, CT v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
 }
 
 // acc[k] += sum over blocks of part[4 b + k], run by a whole block
@@ -219,12 +338,98 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_pull(AdjArgs<CT> a, double* acc,
     const long long n = g.idx(c[0], c[1], c[2]);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int xx = wrap_axis(g, 0, c[0] + L::ex(q)), xy = wrap_axis(g, 1, c[1] + L::ey(q));
-        const int xz = (D == 3) ? wrap_axis(g, 2, c[2] + L::ez(q)) : c[2];
-        const long long x = g.idx(xx, xy, xz);
-        a.lF[(long long)q * g.nodes + n] = a.lfp[(long long)q * g.nodes + x];
-        a.lG[(long long)q * g.nodes + n] = a.lgp[(long long)q * g.nodes + x] + a.lphin[x];
+        // the bulk node that pulls (n, q): n + e_q, wrapped along axes periodic
+        // on both ends; boundary-layer readers come from the row lists
+        int x3[3] = {c[0] + L::ex(q), c[1] + L::ey(q), c[2] + L::ez(q)};
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax)
+            if (a.bc.both_periodic[ax]) x3[ax] = wrap_axis(g, ax, x3[ax]);
+        CT vf = CT(0), vg = CT(0);
+        if (g.interior(x3, D) && !in_shell<D>(g, a.bc, x3)) {
+            const long long x = g.idx(x3[0], x3[1], x3[2]);
+            vf = a.lfp[(long long)q * g.nodes + x];
+            vg = a.lgp[(long long)q * g.nodes + x] + a.lphin[x];
+        }
+        a.lF[(long long)q * g.nodes + n] = vf;
+        a.lG[(long long)q * g.nodes + n] = vg;
     }
+}
+
+// Weight of a boundary-chain term (1, or a free-convect weight from u^n).
+template <class CT>
+__device__ __forceinline__ CT chain_weight(const AdjEntry& e, const CT* u) {
+    if (e.uidx < 0) return CT(1);
+    const CT un = fabs(u[e.uidx]);
+    return (e.branch == 0 ? un : CT(1)) / (CT(1) + un);
+}
+
+// Adjoint reaching one row's target from its boundary-layer readers: f and g
+// parts (g with the phase sum's phi adjoint at the reader).
+template <class CT>
+__device__ __forceinline__ void row_sums(const AdjArgs<CT>& a, const AdjRows& r, int row, CT& sf, CT& sg) {
+    sf = CT(0);
+    sg = CT(0);
+    for (int k = r.start[row]; k < r.start[row + 1]; ++k) {
+        const AdjEntry& e = r.e[k];
+        const CT w = chain_weight(e, a.uo);
+        sf += w * a.lfp[e.rd];
+        sg += w * (a.lgp[e.rd] + a.lphin[e.rd % a.g.nodes]);
+    }
+}
+
+// Boundary-layer readers of state populations: lF / lG += their adjoint.
+template <class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_csr_pop(AdjArgs<CT> a, AdjRows r) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= r.rows) return;
+    CT sf, sg;
+    row_sums(a, r, row, sf, sg);
+    a.lF[r.target[row]] += sf;
+    a.lG[r.target[row]] += sg;
+}
+
+// Readers of outflow-plane entries: the adjoint of P^n (every entry has a
+// row; the target indexes the concatenated plane sets).
+template <class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_csr_prev(AdjArgs<CT> a, AdjRows r) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= r.rows) return;
+    CT sf, sg;
+    row_sums(a, r, row, sf, sg);
+    a.lPf_out[r.target[row]] = sf;
+    a.lPg_out[r.target[row]] = sg;
+}
+
+// Readers of inlet ghost values: accumulated over the sweep (row-indexed).
+template <class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_csr_inlet(AdjArgs<CT> a, AdjRows r, double* accf, double* accg) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= r.rows) return;
+    CT sf, sg;
+    row_sums(a, r, row, sf, sg);
+    accf[row] += (double)sf;
+    accg[row] += (double)sg;
+}
+
+// u^n through the free-convect weights: d/du [(P + |u| X) / (1 + |u|)] =
+// sign(u) (X - P) / (1 + |u|)^2, for f and g readers.
+template <class CT>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_csr_u(AdjArgs<CT> a, AdjRows r) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= r.rows) return;
+    const long long ui = r.target[row];
+    const CT uu = a.uo[ui];
+    const CT un = fabs(uu);
+    const CT sg = uu > CT(0) ? CT(1) : (uu < CT(0) ? CT(-1) : CT(0));
+    const CT k = sg / ((CT(1) + un) * (CT(1) + un));
+    CT acc = CT(0);
+    for (int j = r.start[row]; j < r.start[row + 1]; ++j) {
+        const AdjEntry& e = r.e[j];
+        const CT xf = a.F[e.xidx], xg = a.G[e.xidx];
+        const CT pf = a.pf[a.poff[e.side] + e.pidx], pg = a.pg[a.poff[e.side] + e.pidx];
+        acc += k * ((xf - pf) * a.lfp[e.rd] + (xg - pg) * (a.lgp[e.rd] + a.lphin[e.rd % a.g.nodes]));
+    }
+    a.lU[ui] += acc;
 }
 
 // Reverse of the initial state and first collision at every interior node
@@ -245,10 +450,30 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_init(AdjArgs<CT> a) {
         }
         CT c0, dc[3], lap;
         phi_stencil<L, CT>(a.phin, g, n, c0, dc, lap);
+        // adjoint of the first outflow planes (the initial f / g on outflow
+        // boundary nodes, k_init's store_prev)
+        CT lpf[Q], lpg[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) lpf[q] = lpg[q] = CT(0);
+        if (a.bc.is_fc && a.lPf_in) {
+#pragma unroll
+            for (int side = 0; side < 2 * D; ++side) {
+                const int ax = side >> 1;
+                const int inward = (side & 1) ? -1 : 1;
+                if (a.bc.type[side] != BC_FREE_CONVECT || c[ax] != ((side & 1) ? g.dmax[ax] : g.dmin[ax])) continue;
+                const long long t = plane_index(g, side, c), ps = plane_size(g, side);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (L::e(q, ax) == inward) {
+                        lpf[q] += a.lPf_in[a.poff[side] + (long long)q * ps + t];
+                        lpg[q] += a.lPg_in[a.poff[side] + (long long)q * ps + t];
+                    }
+            }
+        }
         CT lc0 = CT(0), ldc[3] = {CT(0), CT(0), CT(0)}, llap = CT(0), lb = CT(0), lk = CT(0), lm = CT(0),
            lsg = CT(0);
         init_reverse<L, CT>(a.ph, a.sigma, a.half_length, c0, dc, lap, lfs, lgs, lc0, ldc, llap, lb, lk, lm, lsg,
-                            a.nfloor);
+                            a.nfloor, lpf, lpg);
         a.lst[n] = lc0;
 #pragma unroll
         for (int d = 0; d < D; ++d) a.lst[(long long)(1 + d) * g.nodes + n] = ldc[d];

@@ -80,6 +80,10 @@ struct Impl {
     virtual void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every,
                                  std::vector<double>& phi_out) = 0;
     virtual void adjoint_options(double normal_floor) = 0;
+    // Inner checkpoint interval of the reverse sweep (0: one level).
+    virtual void adjoint_nesting(int inner) = 0;
+    // dL/dA_m of the inlet modes after the last adjoint_backward.
+    virtual void adjoint_inlet_grad(std::vector<double>& out) = 0;
     virtual void adjoint_backward(const std::vector<double>& grad_out, std::vector<double>& grad_phi0,
                                   double* grad_sigma, double* grad_mobility) = 0;
     int D = 2, Q = 9;

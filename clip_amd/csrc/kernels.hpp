@@ -78,6 +78,56 @@ struct DevBC {
     const unsigned char* vflag[6];  // per velocity side: 1 = ghost node inside the inlet disc
 };
 
+// Grid of a configuration: CLIP's extents (2D: N + 2 with the ghost ring
+// outside N; 3D: N with the ring inside N, Domain.cu:22-82), interior
+// [1, ext - 2] per axis, and the row stride rounded up to a multiple of
+// `unit` elements (1: dense).
+inline DevGrid grid_of_config(const clip_config& c, int D, long long unit) {
+    DevGrid g{};
+    for (int a = 0; a < 3; ++a) {
+        if (D == 2) {
+            g.ext[a] = (a < 2) ? (int)c.N[a] + 2 : 1;
+            g.dmin[a] = (a < 2) ? 1 : 0;
+            g.dmax[a] = (a < 2) ? g.ext[a] - 2 : 0;
+        } else {
+            g.ext[a] = (int)c.N[a];
+            g.dmin[a] = 1;
+            g.dmax[a] = g.ext[a] - 2;
+        }
+    }
+    g.sy = (g.ext[0] + unit - 1) / unit * unit;
+    g.sz = g.sy * g.ext[1];
+    g.nodes = g.sz * g.ext[2];
+    return g;
+}
+
+// Side types and the launch flags of the reference's boundary classes
+// (Boundary.cu:291-318); the inlet disc flags are set by the solver.
+inline DevBC bc_of_config(const clip_config& c, int D) {
+    DevBC bc{};
+    for (int s = 0; s < 6; ++s) bc.type[s] = (s < 2 * D) ? c.bc_type[s] : -1;
+    bool isWall = false, isFC = false, isNeu = false, isVel = false, isSlip = false;
+    for (int s = 0; s < 2 * D; ++s) {
+        isWall |= c.bc_type[s] == CLIP_BC_WALL;
+        isSlip |= c.bc_type[s] == CLIP_BC_SLIP_WALL;
+        isFC |= c.bc_type[s] == CLIP_BC_FREE_CONVECT;
+        isNeu |= c.bc_type[s] == CLIP_BC_NEUMANN;
+        isVel |= c.bc_type[s] == CLIP_BC_VELOCITY;
+    }
+    for (int a = 0; a < 3; ++a) {
+        const bool lo = a < D && c.bc_type[2 * a] == CLIP_BC_PERIODIC;
+        const bool hi = a < D && c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC;
+        bc.periodic[a] = lo || hi;
+        bc.both_periodic[a] = lo && hi;
+    }
+    bc.is_wall = isWall;
+    bc.is_fc = isFC;
+    bc.is_neumann = isNeu;
+    bc.is_velocity = isVel;
+    bc.mirror_gate = isWall || isFC || isSlip || isNeu;
+    return bc;
+}
+
 // Index of a ghost-plane (or boundary-plane) cell of side `side` within its
 // plane: the two tangential coordinates, lower axis fastest.
 __host__ __device__ inline long long plane_index(const DevGrid& g, int side, const int c[3]) {

@@ -523,10 +523,12 @@ CLIP_HD void node_reverse(const Phys<T>& ph, const T fp[L::Q], const T gp[L::Q],
 // (KernelInitializeDistributions, NsAllen.cu:137-202, then the collision with
 // u = 0 and the initial p): f = p w, g = phi w - w 2 phi (1 - phi) / W (e.n),
 // p = k phi sigma / (L/2) / (rho/3) (k = +1 drop, -1 bubble, 0 otherwise).
+// lpf / lpg (optional): adjoints of the initial f / g themselves (they are
+// also the first outflow planes on free-convect boundary nodes).
 template <class L, class T>
 CLIP_HD void init_reverse(const Phys<T>& ph, T sigma, T half_length, T c0, const T dc[3], T lap, const T lfs[L::Q],
                           const T lgs[L::Q], T& lc0, T ldc[3], T& llap, T& lbeta, T& lkappa, T& lmob, T& lsigma,
-                          T nfloor = T(0)) {
+                          T nfloor = T(0), const T* lpf = nullptr, const T* lpg = nullptr) {
     constexpr int D = L::D;
     constexpr int Q = L::Q;
     const NodeScalars<T> z = node_scalars<L>(ph, c0, lap);
@@ -548,8 +550,9 @@ CLIP_HD void init_reverse(const Phys<T>& ph, T sigma, T half_length, T c0, const
         const T a = lfs[q];
         const T b = lgs[q];
         const T en = edot<L>(q, nr);
-        // f*_q = p w + w h (e.F)
+        // f*_q = p w + w h (e.F); f_q = p w
         lp += a * w;
+        if (lpf) lp += lpf[q] * w;
 #pragma unroll
         for (int d = 0; d < D; ++d) lF[d] += a * w * h * T(L::e(q, d));
         lrho -= a * w * h * edot<L>(q, F) * z.rrho;
@@ -558,7 +561,7 @@ CLIP_HD void init_reverse(const Phys<T>& ph, T sigma, T half_length, T c0, const
         const T heq = c0 * w + w * z.A * en;
         lwc += b * (heq - gin);
         const T hh = z.wc * b;
-        const T lg = (T(1) - z.wc) * b;
+        const T lg = (T(1) - z.wc) * b + (lpg ? lpg[q] : T(0));
         lc += hh * w + lg * w;
         lA += hh * w * en;
         lgi -= lg * w * en;

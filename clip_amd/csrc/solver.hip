@@ -298,6 +298,23 @@ int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* gr
     });
 }
 
+int clip_adjoint_set_nesting(clip_solver* s, int inner_every) {
+    if (!s) return set_error(CLIP_ERR_ARG, "null handle");
+    return guard([&] { s->impl->adjoint_nesting(inner_every); });
+}
+
+int clip_adjoint_inlet_gradient(clip_solver* s, double* grad, int n) {
+    if (!s || !grad) return set_error(CLIP_ERR_ARG, "null argument");
+    if (n < 0) return set_error(CLIP_ERR_ARG, "n must be >= 0");
+    int rc = guard([&] {
+        std::vector<double> gr;
+        s->impl->adjoint_inlet_grad(gr);
+        for (int m = 0; m < n; ++m) grad[m] = m < (int)gr.size() ? gr[m] : 0.0;
+    });
+    if (rc == CLIP_ERR_ARG) return set_error(CLIP_ERR_STATE, clip_last_error());
+    return rc;
+}
+
 int clip_adjoint_set_normal_floor(clip_solver* s, double normal_floor) {
     if (!s) return set_error(CLIP_ERR_ARG, "null handle");
     return guard([&] { s->impl->adjoint_options(normal_floor); });

@@ -17,6 +17,7 @@
 #include "geometry.hpp"
 #include "impl.hpp"
 #include "adjoint.hpp"
+#include "adjoint_maps.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -120,18 +121,16 @@ struct SolverImpl : Impl {
         D = L::D;
         Q = L::Q;
         if (c.dim != L::D) throw ConfigError("lattice/dimension mismatch");
-        // Extents: 2D N+2 with the ghost ring outside N, 3D N with the ghost
-        // ring inside N (Domain.cu:22-82).
-        for (int a = 0; a < 3; ++a) {
-            if (L::D == 2) {
-                g.ext[a] = (a < 2) ? (int)c.N[a] + 2 : 1;
-                g.dmin[a] = (a < 2) ? 1 : 0;
-                g.dmax[a] = (a < 2) ? g.ext[a] - 2 : 0;
-            } else {
-                g.ext[a] = (int)c.N[a];
-                g.dmin[a] = 1;
-                g.dmax[a] = g.ext[a] - 2;
-            }
+        // Row stride padded so that every row's first interior node starts a
+        // 128-byte line in every node array (with the offsets of alloc_grid):
+        // a row's collision output then covers whole lines, which no other
+        // block (on another XCD's L2) shares.  Measured on a 19-plane 16-bit
+        // pull kernel on the 256^3 grid: 3.76 -> 4.24 TB/s.
+        // CLIP_AMD_ROW_ALIGN=0 keeps the dense stride (ext_x).
+        {
+            const char* env = std::getenv("CLIP_AMD_ROW_ALIGN");
+            row_align = !(env && env[0] == '0');
+            g = grid_of_config(c, L::D, row_align ? 128 / (long long)std::min(sizeof(ST), sizeof(CT)) : 1);
         }
         for (int a = 0; a < L::D; ++a)
             if (g.dmax[a] - g.dmin[a] + 1 < 3) throw ConfigError("the interior needs at least 3 nodes per axis");
@@ -144,40 +143,7 @@ struct SolverImpl : Impl {
             tbx = (unsigned)((per + 63) / 64 * 64);
             tby = std::max(1u, (unsigned)(BX * BY) / tbx);
         }
-        // Row stride padded so that every row's first interior node starts a
-        // 128-byte line in every node array (with the offsets of alloc_grid):
-        // a row's collision output then covers whole lines, which no other
-        // block (on another XCD's L2) shares.  Measured on a 19-plane 16-bit
-        // pull kernel on the 256^3 grid: 3.76 -> 4.24 TB/s.
-        // CLIP_AMD_ROW_ALIGN=0 keeps the dense stride (ext_x).
-        {
-            const char* env = std::getenv("CLIP_AMD_ROW_ALIGN");
-            row_align = !(env && env[0] == '0');
-            const long long unit = 128 / (long long)std::min(sizeof(ST), sizeof(CT));
-            g.sy = row_align ? (g.ext[0] + unit - 1) / unit * unit : g.ext[0];
-        }
-        g.sz = g.sy * g.ext[1];
-        g.nodes = g.sz * g.ext[2];
-        for (int s = 0; s < 6; ++s) bc.type[s] = (s < 2 * L::D) ? c.bc_type[s] : -1;
-        bool isWall = false, isFC = false, isNeu = false, isVel = false, isSlip = false;
-        for (int s = 0; s < 2 * L::D; ++s) {
-            isWall |= c.bc_type[s] == CLIP_BC_WALL;
-            isSlip |= c.bc_type[s] == CLIP_BC_SLIP_WALL;
-            isFC |= c.bc_type[s] == CLIP_BC_FREE_CONVECT;
-            isNeu |= c.bc_type[s] == CLIP_BC_NEUMANN;
-            isVel |= c.bc_type[s] == CLIP_BC_VELOCITY;
-        }
-        for (int a = 0; a < 3; ++a) {
-            const bool lo = a < L::D && c.bc_type[2 * a] == CLIP_BC_PERIODIC;
-            const bool hi = a < L::D && c.bc_type[2 * a + 1] == CLIP_BC_PERIODIC;
-            bc.periodic[a] = lo || hi;
-            bc.both_periodic[a] = lo && hi;
-        }
-        bc.is_wall = isWall;
-        bc.is_fc = isFC;
-        bc.is_neumann = isNeu;
-        bc.is_velocity = isVel;
-        bc.mirror_gate = isWall || isFC || isSlip || isNeu;
+        bc = bc_of_config(c, L::D);
         ph.rho_h = (CT)c.rho_h;
         ph.rho_l = (CT)c.rho_l;
         ph.tau_h = (CT)c.tau_h;
@@ -1026,28 +992,50 @@ struct SolverImpl : Impl {
     }
 
     // ------------------------------------------------------------- adjoint
-    // Reverse mode over nsteps steps from a given interior phi0, with states
-    // kept every `every` steps and each segment recomputed in the sweep.
+    // Reverse mode over nsteps steps from a given interior phi0.  The forward
+    // keeps a state every `every` steps; the sweep recomputes each segment.
+    // With an inner interval (0 < inner < every) the recomputation keeps a
+    // state every `inner` steps of the segment and recomputes each of those
+    // sub-segments once more (two checkpoint levels: memory every/inner +
+    // inner states instead of every, one more forward pass).
     std::vector<void*> adj_allocs;
-    size_t adj_capacity = 0;  // CT elements in the state pool
+    size_t adj_capacity = 0;  // states in the pool
+    size_t adj_bytes = 0;
     CT* adj_pool = nullptr;
     CT *lF = nullptr, *lG = nullptr, *lU = nullptr, *lPhi = nullptr, *lfp = nullptr, *lgp = nullptr,
        *lst = nullptr, *lphin = nullptr;
-    double *adj_part = nullptr, *adj_acc = nullptr;
-    int adj_T = 0, adj_K = 0, adj_nsnap = 0;
+    CT* lP[2] = {nullptr, nullptr};  // adjoint of the outflow planes (f sides, then g sides), ping-pong
+    int lpi = 0;                     // lP[lpi] holds the adjoint of the planes of the state after the step
+    double *adj_part = nullptr, *adj_acc = nullptr, *in_accf = nullptr, *in_accg = nullptr;
+    int adj_T = 0, adj_K = 0, adj_inner = 0, adj_nsnap = 0, adj_nmid = 0, adj_leaf = 0;
     double adj_nfloor = 0.0;
-    bool adj_ready = false;
+    bool adj_ready = false, adj_grad_ready = false;
+    std::vector<double> adj_inlet_grad;
+    std::vector<CT> adj_phi0;  // phi of the initial state on every node (device layout)
+    // host-built transposes (once per handle)
+    bool maps_ready = false;
+    AdjRows rows_pop{}, rows_prev{}, rows_inlet{}, rows_u{}, rows_ring{}, rows_init{};
+    std::vector<long long> inlet_rows;  // population index (q*nodes + ghost node) of each inlet row
+    long long poff[6] = {};
+    long long prev_elems = 0;           // entries of one plane set over all free-convect sides
 
-    size_t state_elems() const { return (size_t)g.nodes * (2 * QL + 1 + L::D); }
-    // pool: kept states 0, K, 2K, ... | the final state T | segment states
-    CT* snap_ptr(int k) { return adj_pool + (size_t)k * state_elems(); }
-    CT* final_ptr() { return adj_pool + (size_t)adj_nsnap * state_elems(); }
-    CT* seg_ptr(int j) { return adj_pool + (size_t)(adj_nsnap + 1 + j) * state_elems(); }
-    // parts of a kept state: f*, g*, phi (with its ghost ring), u
+    size_t state_elems() const { return (size_t)g.nodes * (2 * QL + 1 + L::D) + 2 * (size_t)prev_elems; }
+    // pool: kept states 0, K, 2K, ... | the final state | inner-level states | leaf states
+    CT* slot(size_t k) { return adj_pool + k * state_elems(); }
+    CT* snap_ptr(int k) { return slot((size_t)k); }
+    CT* final_ptr() { return slot((size_t)adj_nsnap); }
+    CT* mid_ptr(int j) { return slot((size_t)(adj_nsnap + 1 + j)); }
+    CT* leaf_ptr(int j) { return slot((size_t)(adj_nsnap + 1 + adj_nmid + j)); }
+    size_t pool_states() const {
+        return (size_t)adj_nsnap + 1 + adj_nmid + (size_t)std::max(adj_leaf - 1, 2);
+    }
+    // parts of a kept state: f*, g*, phi (with its ghost ring), u, outflow planes
     CT* st_f(CT* st) const { return st; }
     CT* st_g(CT* st) const { return st + (size_t)g.nodes * QL; }
     CT* st_phi(CT* st) const { return st + (size_t)g.nodes * 2 * QL; }
     CT* st_u(CT* st) const { return st + (size_t)g.nodes * (2 * QL + 1); }
+    CT* st_pf(CT* st) const { return st + (size_t)g.nodes * (2 * QL + 1 + L::D); }
+    CT* st_pg(CT* st) const { return st_pf(st) + prev_elems; }
 
     // One forward step from kept state `a` into kept state `b` (no copies).
     void step_into(CT* a, CT* b) {
@@ -1061,6 +1049,13 @@ struct SolverImpl : Impl {
             s_.ufc = st_u(a);
             s_.ucol = st_u(a);
             s_.uout = st_u(b);
+            for (int s = 0; s < 6; ++s) {
+                const bool fc = s < 2 * L::D && cfg.bc_type[s] == CLIP_BC_FREE_CONVECT;
+                s_.prev_f_in[s] = fc ? st_pf(a) + poff[s] : nullptr;
+                s_.prev_g_in[s] = fc ? st_pg(a) + poff[s] : nullptr;
+                s_.prev_f_out[s] = fc ? st_pf(b) + poff[s] : nullptr;
+                s_.prev_g_out[s] = fc ? st_pg(b) + poff[s] : nullptr;
+            }
             ghost_refresh(st_phi(a), st_phi(b));
             launch_phase_pull(s_);
             launch_fused_pull<MODE_FULL>(s_);
@@ -1082,7 +1077,8 @@ struct SolverImpl : Impl {
         if constexpr (!std::is_same<ST, CT>::value)
             throw ConfigError("the adjoint runs on F64 or F32 handles");
         for (int s = 0; s < 2 * L::D; ++s)
-            if (cfg.bc_type[s] != CLIP_BC_PERIODIC) throw ConfigError("the adjoint needs a fully periodic domain");
+            if (cfg.bc_type[s] == CLIP_BC_SLIP_WALL && !bc.is_wall)
+                throw ConfigError("the adjoint does not support a slip side without a wall side");
     }
     template <class T>
     T* adj_alloc(size_t count) {
@@ -1092,37 +1088,140 @@ struct SolverImpl : Impl {
             throw std::bad_alloc();
         }
         adj_allocs.push_back(ptr);
+        adj_bytes += count * sizeof(T);
+        bytes += count * sizeof(T);
         return (T*)ptr;
     }
     void adj_free() {
         for (void* p2 : adj_allocs) (void)hipFree(p2);
         adj_allocs.clear();
+        bytes -= adj_bytes;
+        adj_bytes = 0;
         adj_capacity = 0;
         adj_pool = nullptr;
+        maps_ready = false;
     }
     void save_state(CT* dst) {
         static_assert(kPlain, "plain storage only");
         const size_t nq = (size_t)g.nodes * QL;
         HIP_CHECK(hipMemcpyAsync(dst, fbuf[cur], nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(dst + nq, gbuf[cur], nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(dst + 2 * nq, phib[phc], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice,
+        HIP_CHECK(hipMemcpyAsync(st_phi(dst), phib[phc], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice,
                                  stream));
-        HIP_CHECK(hipMemcpyAsync(dst + 2 * nq + g.nodes, u[ucur], (size_t)g.nodes * L::D * sizeof(CT),
-                                 hipMemcpyDeviceToDevice, stream));
-    }
-    void load_state(const CT* src) {
-        const size_t nq = (size_t)g.nodes * QL;
-        HIP_CHECK(hipMemcpyAsync(fbuf[cur], src, nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(gbuf[cur], src + nq, nq * sizeof(CT), hipMemcpyDeviceToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(phib[phc], src + 2 * nq, (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice,
+        HIP_CHECK(hipMemcpyAsync(st_u(dst), u[ucur], (size_t)g.nodes * L::D * sizeof(CT), hipMemcpyDeviceToDevice,
                                  stream));
-        HIP_CHECK(hipMemcpyAsync(u[ucur], src + 2 * nq + g.nodes, (size_t)g.nodes * L::D * sizeof(CT),
-                                 hipMemcpyDeviceToDevice, stream));
-        direct_old = false;
-        old_ref_zero = false;
+        for (int s = 0; s < 2 * L::D; ++s) {
+            if (cfg.bc_type[s] != CLIP_BC_FREE_CONVECT) continue;
+            const size_t n = (size_t)plane_size(g, s) * QL * sizeof(CT);
+            HIP_CHECK(hipMemcpyAsync(st_pf(dst) + poff[s], prevf[pcur][s], n, hipMemcpyDeviceToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(st_pg(dst) + poff[s], prevg[pcur][s], n, hipMemcpyDeviceToDevice, stream));
+        }
     }
 
     static constexpr bool kPlain = std::is_same<ST, CT>::value;
+
+    bool shell_host(const int c[3]) const {
+        bool s = false;
+        for (int a = 0; a < L::D; ++a) s |= !bc.both_periodic[a] && (c[a] == g.dmin[a] || c[a] == g.dmax[a]);
+        return s;
+    }
+    std::vector<std::vector<unsigned char>> inlet_flags_host() const { return inlet_flags(cfg, g, L::D); }
+    // (target, entry) pairs -> row lists in device memory (rows by target,
+    // entries by reader: a fixed order)
+    AdjRows upload_rows(std::vector<std::pair<long long, AdjEntry>>& v, std::vector<long long>* targets = nullptr) {
+        std::sort(v.begin(), v.end(), [](const std::pair<long long, AdjEntry>& x, const std::pair<long long, AdjEntry>& y) {
+            return x.first != y.first ? x.first < y.first : x.second.rd < y.second.rd;
+        });
+        std::vector<long long> tg;
+        std::vector<int> st;
+        std::vector<AdjEntry> en;
+        for (size_t i = 0; i < v.size(); ++i) {
+            if (i == 0 || v[i].first != v[i - 1].first) {
+                tg.push_back(v[i].first);
+                st.push_back((int)en.size());
+            }
+            en.push_back(v[i].second);
+        }
+        st.push_back((int)en.size());
+        if (targets) *targets = tg;
+        AdjRows r{};
+        r.rows = (int)tg.size();
+        if (!r.rows) return r;
+        long long* dt = adj_alloc<long long>(tg.size());
+        int* ds = adj_alloc<int>(st.size());
+        AdjEntry* de = adj_alloc<AdjEntry>(en.size());
+        HIP_CHECK(hipMemcpy(dt, tg.data(), tg.size() * sizeof(long long), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(ds, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(de, en.data(), en.size() * sizeof(AdjEntry), hipMemcpyHostToDevice));
+        r.target = dt;
+        r.start = ds;
+        r.e = de;
+        return r;
+    }
+    // Transposes of the boundary-layer pulls, the phi ring copies and the
+    // initial ring (SymPuller, ring_sources).
+    void maps_build() {
+        if (maps_ready) return;
+        const auto vf = inlet_flags_host();
+        SymPuller<L> sp{g, bc, vf};
+        std::vector<std::pair<long long, AdjEntry>> pop, prv, inl, uu;
+        std::vector<PullTerm> terms;
+        for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
+            for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
+                for (int i = g.dmin[0]; i <= g.dmax[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    if (!shell_host(c)) continue;
+                    const long long x = g.idx(i, j, k);
+                    for (int q = 0; q < QL; ++q) {
+                        terms.clear();
+                        sp.lvl3(c, q, terms);
+                        const long long rd = (long long)q * g.nodes + x;
+                        const PullTerm* X = nullptr;
+                        const PullTerm* P = nullptr;
+                        for (const auto& t : terms) {
+                            AdjEntry e{rd, t.fc >= 0 ? (long long)(t.fc / 2) * g.nodes + t.nb : -1, -1, -1, t.branch, 0};
+                            if (t.kind == 0) pop.push_back({t.idx, e});
+                            else if (t.kind == 1) prv.push_back({poff[t.side] + t.idx, e});
+                            else if (t.kind == 2 || t.kind == 3) inl.push_back({t.idx, e});
+                            if (t.fc >= 0 && t.kind == 1) P = &t;
+                            if (t.fc >= 0 && t.kind != 1) X = &t;
+                        }
+                        if (P) {
+                            if (!X) throw ConfigError("adjoint: free-convect pull without a neighbour value");
+                            uu.push_back({(long long)(P->fc / 2) * g.nodes + P->nb,
+                                          AdjEntry{rd, -1, X->idx, P->idx, 0, P->side}});
+                        }
+                    }
+                }
+        rows_pop = upload_rows(pop);
+        rows_prev = upload_rows(prv);
+        rows_inlet = upload_rows(inl, &inlet_rows);
+        rows_u = upload_rows(uu);
+        // ring copies of one step, and of the initial phi (ghosts wrapped
+        // along periodic axes copy phi0; others are constants)
+        const std::vector<long long> src = ring_sources(g, bc, L::D, vf);
+        std::vector<std::pair<long long, AdjEntry>> ring, init;
+        for (long long n = 0; n < g.nodes; ++n)
+            if (src[n] >= 0) ring.push_back({src[n], AdjEntry{n, -1, -1, -1, 0, 0}});
+        for (int k = 0; k < g.ext[2]; ++k)
+            for (int j = 0; j < g.ext[1]; ++j)
+                for (int i = 0; i < g.ext[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    if (g.interior(c, L::D)) continue;
+                    int w[3] = {i, j, k};
+                    bool wrapped = true;
+                    for (int a = 0; a < L::D; ++a) {
+                        if (w[a] >= g.dmin[a] && w[a] <= g.dmax[a]) continue;
+                        if (!bc.periodic[a]) wrapped = false;
+                        const int span = g.dmax[a] - g.dmin[a] + 1;
+                        w[a] += (w[a] < g.dmin[a]) ? span : -span;
+                    }
+                    if (wrapped) init.push_back({g.idx(w[0], w[1], w[2]), AdjEntry{g.idx(i, j, k), -1, -1, -1, 0, 0}});
+                }
+        rows_ring = upload_rows(ring);
+        rows_init = upload_rows(init);
+        maps_ready = true;
+    }
 
     void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every, std::vector<double>& out) override {
         adj_check();
@@ -1131,7 +1230,9 @@ struct SolverImpl : Impl {
     void adjoint_forward_impl(const std::vector<double>& phi0, int nsteps, int every, std::vector<double>& out) {
         if (nsteps < 1 || every < 1) throw ConfigError("adjoint: nsteps and checkpoint interval must be >= 1");
         if ((long long)phi0.size() != interior_count()) throw ConfigError("adjoint: phi0 size != interior nodes");
-        // phi on every node: interior from phi0, ghosts by periodic wrap
+        // phi on every node: interior from phi0; a ghost cell copies phi0 at
+        // its wrapped position when every axis it lies outside of is periodic,
+        // else it keeps the case's initial phase field (a constant)
         std::vector<CT> h((size_t)g.nodes, CT(0));
         for_interior([&](long long t, long long n) { h[n] = (CT)phi0[t]; });
         for (int k = 0; k < g.ext[2]; ++k)
@@ -1140,18 +1241,34 @@ struct SolverImpl : Impl {
                     const int c[3] = {i, j, k};
                     if (g.interior(c, L::D)) continue;
                     int w[3] = {i, j, k};
+                    bool wrapped = true;
                     for (int a = 0; a < L::D; ++a) {
+                        if (w[a] >= g.dmin[a] && w[a] <= g.dmax[a]) continue;
+                        if (!bc.periodic[a]) wrapped = false;
                         const int span = g.dmax[a] - g.dmin[a] + 1;
-                        if (w[a] < g.dmin[a]) w[a] += span;
-                        if (w[a] > g.dmax[a]) w[a] -= span;
+                        w[a] += (w[a] < g.dmin[a]) ? span : -span;
                     }
-                    h[g.idx(i, j, k)] = h[g.idx(w[0], w[1], w[2])];
+                    h[g.idx(i, j, k)] = wrapped ? h[g.idx(w[0], w[1], w[2])] : (CT)initial_phi(cfg, i, j, k);
                 }
-        const int nsnap = (nsteps + every - 1) / every;
-        const size_t need = (size_t)(nsnap + 1 + every) * state_elems();
-        if (need > adj_capacity) {
+        prev_elems = 0;
+        for (int s = 0; s < 6; ++s) {
+            poff[s] = prev_elems;
+            if (s < 2 * L::D && cfg.bc_type[s] == CLIP_BC_FREE_CONVECT) prev_elems += plane_size(g, s) * QL;
+        }
+        adj_T = nsteps;
+        adj_K = every;
+        adj_nsnap = (nsteps + every - 1) / every;
+        if (adj_inner > 0 && adj_inner < every) {
+            adj_leaf = adj_inner;
+            adj_nmid = (every + adj_inner - 1) / adj_inner - 1;
+        } else {
+            adj_leaf = every;
+            adj_nmid = 0;
+        }
+        const size_t need = pool_states();
+        if (need > adj_capacity || !adj_pool) {
             adj_free();
-            adj_pool = adj_alloc<CT>(need);
+            adj_pool = adj_alloc<CT>(need * state_elems());
             adj_capacity = need;
             const size_t nq = (size_t)g.nodes * QL;
             lF = adj_alloc<CT>(nq);
@@ -1162,15 +1279,26 @@ struct SolverImpl : Impl {
             lPhi = adj_alloc<CT>((size_t)g.nodes);
             lphin = adj_alloc<CT>((size_t)g.nodes);
             lst = adj_alloc<CT>((size_t)g.nodes * (L::D + 2));
+            if (prev_elems) {
+                lP[0] = adj_alloc<CT>(2 * (size_t)prev_elems);
+                lP[1] = adj_alloc<CT>(2 * (size_t)prev_elems);
+            } else {
+                lP[0] = lP[1] = nullptr;
+            }
             const long long blocks = (interior_count() + ADJ_TB - 1) / ADJ_TB;
             adj_part = adj_alloc<double>((size_t)blocks * 4);
             adj_acc = adj_alloc<double>(4);
+            maps_build();
+            in_accf = adj_alloc<double>(std::max<size_t>(1, inlet_rows.size()));
+            in_accg = adj_alloc<double>(std::max<size_t>(1, inlet_rows.size()));
         }
-        adj_T = nsteps;
-        adj_K = every;
-        adj_nsnap = nsnap;
+        adj_phi0 = h;
         init_with_phi(h);
         save_state(snap_ptr(0));
+        // every other slot starts as a copy of the initial state: cells no
+        // step writes (ghost populations, constant ring cells) hold their values
+        for (size_t k = 1; k < need; ++k)
+            HIP_CHECK(hipMemcpyAsync(slot(k), slot(0), state_elems() * sizeof(CT), hipMemcpyDeviceToDevice, stream));
         for (int n = 1; n <= nsteps; ++n) {
             step(1);
             if (n % every == 0 && n < nsteps) save_state(snap_ptr(n / every));
@@ -1181,18 +1309,20 @@ struct SolverImpl : Impl {
         out.assign((size_t)interior_count(), 0.0);
         for_interior([&](long long t, long long n) { out[t] = phi[n]; });
         adj_ready = true;
+        adj_grad_ready = false;
         step_count = nsteps;
     }
 
-    // Arguments of the reverse of the step from kept state `st` (phin: phi of
-    // the state after it, i.e. the step's stencil input with its lagged ring).
-    AdjArgs<CT> adj_args(CT* st, const CT* phin) {
+    // Arguments of the reverse of the step from kept state `st` to the state
+    // `after` (whose phi is the step's stencil input with its lagged ring).
+    AdjArgs<CT> adj_args(CT* st, CT* after) {
         AdjArgs<CT> A;
         A.g = g;
+        A.bc = bc;
         A.ph = ph;
         A.F = st_f(st);
         A.G = st_g(st);
-        A.phin = phin;
+        A.phin = st_phi(after);
         A.uo = st_u(st);
         A.lF = lF;
         A.lG = lG;
@@ -1206,26 +1336,89 @@ struct SolverImpl : Impl {
         A.sigma = (CT)cfg.sigma;
         A.half_length = (CT)(cfg.reference_length / 2.0);
         A.nfloor = (CT)adj_nfloor;
+        A.pf = prev_elems ? st_pf(st) : nullptr;
+        A.pg = prev_elems ? st_pg(st) : nullptr;
+        A.lPf_in = lP[lpi];
+        A.lPg_in = lP[lpi] ? lP[lpi] + prev_elems : nullptr;
+        A.lPf_out = lP[1 - lpi];
+        A.lPg_out = lP[1 - lpi] ? lP[1 - lpi] + prev_elems : nullptr;
+        for (int s = 0; s < 6; ++s) A.poff[s] = poff[s];
         return A;
     }
 
-    // Reverse of the step from kept state `st` to the state whose phi is
-    // `phin`.  The partial sums of the parameter adjoints are folded into
-    // the accumulator by k_adj_pull's extra last block.
-    void adjoint_step(CT* st, const CT* phin) {
-        const AdjArgs<CT> A = adj_args(st, phin);
+    void fill(CT* x, long long n) {
+        if (n <= 0) return;
+        k_adj_fill<CT><<<(unsigned)((n + ADJ_TB - 1) / ADJ_TB), ADJ_TB, 0, stream>>>(x, n, CT(0));
+        launch_check("k_adj_fill");
+    }
+    template <class Fn>
+    void launch_rows(const AdjRows& r, Fn fn) {
+        if (r.rows) fn((unsigned)((r.rows + ADJ_TB - 1) / ADJ_TB));
+    }
+
+    // Reverse of the step from kept state `st` into `after`.  The partial sums
+    // of the parameter adjoints are folded into the accumulator by
+    // k_adj_pull's extra last block.
+    void adjoint_step(CT* st, CT* after) {
+        const AdjArgs<CT> A = adj_args(st, after);
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
+        const unsigned nb = (unsigned)((g.nodes + ADJ_TB - 1) / ADJ_TB);
+        if (shell_blocks()) {
+            k_adj_shell_post<L, CT><<<dim3(shell_blocks(), QL), ADJ_TB, 0, stream>>>(A, make_shell<L>(g, bc));
+            launch_check("k_adj_shell_post");
+        }
         k_adj_node<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_node");
-        k_adj_phi<L, CT, true><<<blocks, ADJ_TB, 0, stream>>>(A);
+        k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
         launch_check("k_adj_phi");
+        fill(lPhi, g.nodes);
+        launch_rows(rows_ring, [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_ring, lphin, lPhi); });
         k_adj_pull<L, CT><<<blocks + 1, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
         launch_check("k_adj_pull");
+        launch_rows(rows_pop, [&](unsigned b) { k_adj_csr_pop<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_pop); });
+        if (prev_elems) {
+            fill(lP[1 - lpi], 2 * prev_elems);
+            launch_rows(rows_prev, [&](unsigned b) { k_adj_csr_prev<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_prev); });
+        }
+        launch_rows(rows_inlet,
+                    [&](unsigned b) { k_adj_csr_inlet<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_inlet, in_accf, in_accg); });
+        launch_rows(rows_u, [&](unsigned b) { k_adj_csr_u<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_u); });
+        launch_check("k_adj_csr");
+        lpi = 1 - lpi;
+    }
+
+    // Steps s0 .. s1-1 in reverse, from kept state `sa` (state s0); `after`
+    // is state s1.  Leaf level: every state recomputed into the leaf slots.
+    void reverse_leaf(CT* sa, int s0, int s1, CT* after) {
+        auto state = [&](int n) -> CT* { return n == s0 ? sa : leaf_ptr(n - s0 - 1); };
+        for (int n = s0 + 1; n < s1; ++n) step_into(state(n - 1), state(n));
+        for (int n = s1 - 1; n >= s0; --n) adjoint_step(state(n), n + 1 < s1 ? state(n + 1) : after);
+    }
+    void reverse_segment(CT* sa, int s0, int s1, CT* after) {
+        if (adj_nmid == 0 || s1 - s0 <= adj_leaf) return reverse_leaf(sa, s0, s1, after);
+        // inner level: recompute the segment keeping every adj_leaf-th state
+        // (leaf slots 0 / 1 carry the states in between)
+        CT* x = sa;
+        for (int n = s0 + 1; n < s1; ++n) {
+            const int r = n - s0;
+            CT* dst = (r % adj_leaf == 0) ? mid_ptr(r / adj_leaf - 1) : leaf_ptr(n & 1);
+            step_into(x, dst);
+            x = dst;
+        }
+        const int chunks = (s1 - s0 + adj_leaf - 1) / adj_leaf;
+        for (int c = chunks - 1; c >= 0; --c) {
+            const int c0 = s0 + c * adj_leaf, c1 = std::min(s1, c0 + adj_leaf);
+            reverse_leaf(c == 0 ? sa : mid_ptr(c - 1), c0, c1, c1 < s1 ? mid_ptr(c) : after);
+        }
     }
 
     void adjoint_options(double normal_floor) override {
         if (!(normal_floor >= 0.0)) throw ConfigError("adjoint: normal floor must be >= 0");
         adj_nfloor = normal_floor;
+    }
+    void adjoint_nesting(int inner) override {
+        if (inner < 0) throw ConfigError("adjoint: the inner checkpoint interval must be >= 0");
+        adj_inner = inner;
     }
 
     void adjoint_backward(const std::vector<double>& gout, std::vector<double>& gphi0, double* gsigma,
@@ -1238,51 +1431,162 @@ struct SolverImpl : Impl {
         if (!adj_ready) throw ConfigError("adjoint: call the adjoint forward first");
         if ((long long)gout.size() != interior_count()) throw ConfigError("adjoint: gradient size != interior nodes");
         const size_t nq = (size_t)g.nodes * QL;
-        HIP_CHECK(hipMemsetAsync(lF, 0, nq * sizeof(CT), stream));
-        HIP_CHECK(hipMemsetAsync(lG, 0, nq * sizeof(CT), stream));
-        HIP_CHECK(hipMemsetAsync(lU, 0, (size_t)g.nodes * L::D * sizeof(CT), stream));
-        HIP_CHECK(hipMemsetAsync(lst, 0, (size_t)g.nodes * (L::D + 2) * sizeof(CT), stream));
-        HIP_CHECK(hipMemsetAsync(lphin, 0, (size_t)g.nodes * sizeof(CT), stream));
+        fill(lF, (long long)nq);
+        fill(lG, (long long)nq);
+        fill(lU, g.nodes * L::D);
+        fill(lst, g.nodes * (L::D + 2));
+        fill(lphin, g.nodes);
+        lpi = 0;
+        if (prev_elems) {
+            fill(lP[0], 2 * prev_elems);
+            fill(lP[1], 2 * prev_elems);
+        }
         HIP_CHECK(hipMemsetAsync(adj_acc, 0, 4 * sizeof(double), stream));
+        HIP_CHECK(hipMemsetAsync(in_accf, 0, std::max<size_t>(1, inlet_rows.size()) * sizeof(double), stream));
+        HIP_CHECK(hipMemsetAsync(in_accg, 0, std::max<size_t>(1, inlet_rows.size()) * sizeof(double), stream));
         {
             std::vector<double> lp((size_t)g.nodes, 0.0);
             for_interior([&](long long t, long long n) { lp[n] = gout[t]; });
             upload(lPhi, lp);
         }
-        // segment k: states s0..s1-1 recomputed from the kept state s0 straight
-        // into the segment slots, then swept in reverse; the phi of state n+1
-        // is the stencil input of step n (its ghost ring is the lagged one)
         for (int k = adj_nsnap - 1; k >= 0; --k) {
             const int s0 = k * adj_K;
             const int s1 = std::min(adj_T, s0 + adj_K);
-            auto state = [&](int n) -> CT* { return n == s0 ? snap_ptr(k) : seg_ptr(n - s0); };
-            for (int n = s0 + 1; n < s1; ++n) step_into(state(n - 1), state(n));
-            for (int n = s1 - 1; n >= s0; --n) {
-                CT* after = (n + 1 < s1) ? state(n + 1) : (s1 < adj_T ? snap_ptr(k + 1) : final_ptr());
-                adjoint_step(state(n), st_phi(after));
-            }
+            reverse_segment(snap_ptr(k), s0, s1, s1 < adj_T ? snap_ptr(k + 1) : final_ptr());
         }
-        // initial state and first collision (phi0 with its periodic ring)
-        const AdjArgs<CT> A = adj_args(snap_ptr(0), st_phi(snap_ptr(0)));
+        // initial state and first collision
+        const AdjArgs<CT> A = adj_args(snap_ptr(0), snap_ptr(0));
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
+        const unsigned nb = (unsigned)((g.nodes + ADJ_TB - 1) / ADJ_TB);
         k_adj_init<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_init");
         k_adj_accumulate<CT><<<1, ADJ_TB, 0, stream>>>(adj_part, (int)blocks, adj_acc);
         launch_check("k_adj_accumulate");
-        k_adj_phi<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A);
+        k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
         launch_check("k_adj_phi");
-        std::vector<double> lp;
-        download(lphin, (size_t)g.nodes, lp);
+        fill(lPhi, g.nodes);
+        launch_rows(rows_init, [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_init, lphin, lPhi); });
+        std::vector<double> la, lr;
+        download(lphin, (size_t)g.nodes, la);
+        download(lPhi, (size_t)g.nodes, lr);
         gphi0.assign((size_t)interior_count(), 0.0);
-        for_interior([&](long long t, long long n) { gphi0[t] = lp[n]; });
+        for_interior([&](long long t, long long n) { gphi0[t] = la[n] + lr[n]; });
         double acc[4];
         HIP_CHECK(hipMemcpyAsync(acc, adj_acc, sizeof(acc), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         // beta = 8 sigma / W, kappa = 1.5 sigma W (InputData.cu:76-114)
-        if (gsigma) *gsigma = acc[0] * 8.0 / cfg.interface_width + acc[1] * 1.5 * cfg.interface_width + acc[3];
+        const double gs_ghost = ghost_gradients();
+        if (gsigma)
+            *gsigma = acc[0] * 8.0 / cfg.interface_width + acc[1] * 1.5 * cfg.interface_width + acc[3] + gs_ghost;
         if (gmob) *gmob = acc[2];
         // leave the handle at the final state of the forward run
         adj_ready = false;
+        adj_grad_ready = true;
+    }
+
+    // Gradients through the ghost values the pulls read (accumulated over the
+    // sweep per row of the ghost list).  Inside an inlet disc: dL/dA_m of the
+    // inlet modulation (below).  Elsewhere (do-nothing sides): the initial
+    // state's f = p w with the Laplace pressure p = +-phi sigma / (L/2) /
+    // (rho/3) of the drop / bubble cases (KernelInitializeDistributions,
+    // NsAllen.cu:155-168), whose sigma-derivative is returned; g = phi w there
+    // (the normal is 0 at ghosts) depends on no parameter.
+    //
+    // dL/dA_m of the inlet modulation u_in = value (1 + sum_m A_m cos((m+1) theta
+    // + phase_m)) from the adjoints of the inlet ghost values: those values
+    // (JetBoundary, Solver.cu:578-619) are, with G = Gamma(U) and the ghost
+    // cell's constant populations f, g and phi,
+    //   f_a = f_b + G_a - G_b - (1/weight) e_a,t . M_t,  M_t = sum_{q>=1, e_n=0} e_q,t (f_q - G_q)
+    //   g_a = phi (G_a + w_a) - (g_b - phi (G_b + w_b)) - [a not normal] N / weight,
+    //         N = sum_{q>=1, e_n=0} (g_q - phi (G_q + w_q))
+    // with b = opp(a), so their U-derivatives need only dG_q/dU = w_q (3 e_q +
+    // 9 (e_q.U) e_q - 3 U).
+    double ghost_gradients() {
+        adj_inlet_grad.assign(CLIP_MAX_INLET_MODES, 0.0);
+        if (inlet_rows.empty()) return 0.0;
+        std::vector<double> af(inlet_rows.size()), ag(inlet_rows.size());
+        HIP_CHECK(hipMemcpyAsync(af.data(), in_accf, af.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(ag.data(), in_accg, ag.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        const auto vf = inlet_flags_host();
+        const double ks = cfg.case_type == CLIP_CASE_DROP ? 1.0 : (cfg.case_type == CLIP_CASE_BUBBLE ? -1.0 : 0.0);
+        const double half_length = cfg.reference_length / 2.0;
+        double gsig = 0.0;
+        for (size_t r = 0; r < inlet_rows.size(); ++r) {
+            const long long n = inlet_rows[r] % g.nodes;
+            const int qa = (int)(inlet_rows[r] / g.nodes);
+            int c[3];
+            g.coords(n, c);
+            int side = -1;
+            for (int s = 2 * L::D - 1; s >= 0; --s) {
+                const int a = s >> 1;
+                if (cfg.bc_type[s] != CLIP_BC_VELOCITY) continue;
+                if (c[a] == ((s & 1) ? g.dmax[a] + 1 : g.dmin[a] - 1)) {
+                    if (vf[s][plane_index(g, s, c)]) side = s;
+                    break;
+                }
+            }
+            if (side < 0) {
+                const double ph0 = (double)adj_phi0[n];
+                const double rho = cfg.rho_l + ph0 * (cfg.rho_h - cfg.rho_l);
+                gsig += af[r] * ks * ph0 / half_length / (rho / 3.0) * L::w(qa);
+                continue;
+            }
+            const int ax = side >> 1;
+            double U[3];
+            inlet_velocity(cfg, side, c, U);
+            auto dG = [&](int q, double out[3]) {
+                double eu = 0.0;
+                for (int d = 0; d < L::D; ++d) eu += L::e(q, d) * U[d];
+                for (int d = 0; d < 3; ++d)
+                    out[d] = d < L::D ? L::w(q) * (3.0 * L::e(q, d) + 9.0 * eu * L::e(q, d) - 3.0 * U[d]) : 0.0;
+            };
+            const int b = L::opp(qa);
+            const double weight = (double)(L::NIN - 1);
+            const double phig = (double)adj_phi0[n];
+            double da[3], db[3], dfa[3], dga[3], sumt[3] = {0, 0, 0}, sum0[3] = {0, 0, 0};
+            dG(qa, da);
+            dG(b, db);
+            for (int q = 1; q < QL; ++q) {
+                if (L::e(q, ax) != 0) continue;
+                double dq[3];
+                dG(q, dq);
+                double et = 0.0;  // e_a,t . e_q,t over the tangential axes
+                for (int t = 0; t < L::D; ++t)
+                    if (t != ax) et += (double)L::e(qa, t) * L::e(q, t);
+                for (int d = 0; d < 3; ++d) {
+                    sumt[d] += et * dq[d];
+                    sum0[d] += dq[d];
+                }
+            }
+            bool normal = true;
+            for (int t = 0; t < L::D; ++t)
+                if (t != ax && L::e(qa, t) != 0) normal = false;
+            for (int d = 0; d < 3; ++d) {
+                dfa[d] = da[d] - db[d] + sumt[d] / weight;
+                dga[d] = phig * (da[d] + db[d]) + (normal ? 0.0 : phig * sum0[d] / weight);
+            }
+            // dU / dA_m = value cos((m+1) theta + phase_m) about geometry 0's centre
+            int t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+            if (L::D == 2) {
+                t1 = 1 - ax;
+                t2 = -1;
+            }
+            const double d1 = c[t1] - cfg.geom_center[0][t1];
+            const double d2 = (t2 >= 0) ? c[t2] - cfg.geom_center[0][t2] : 0.0;
+            const double th = std::atan2(d2, d1);
+            for (int m = 0; m < cfg.inlet_modes && m < CLIP_MAX_INLET_MODES; ++m) {
+                const double cm = std::cos((m + 1) * th + cfg.inlet_mode_phase[m]);
+                double s = 0.0;
+                for (int d = 0; d < L::D; ++d) s += (af[r] * dfa[d] + ag[r] * dga[d]) * cfg.bc_value[side][d] * cm;
+                adj_inlet_grad[m] += s;
+            }
+        }
+        return gsig;
+    }
+    void adjoint_inlet_grad(std::vector<double>& out) override {
+        if (!adj_grad_ready) throw ConfigError("adjoint: no backward sweep has run");
+        out = adj_inlet_grad;
     }
 
     void uold_soa(std::vector<double>& out) override { download_field(u[1 - ucur], L::D, out); }

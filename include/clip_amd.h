@@ -252,16 +252,22 @@ int clip_write_vts(clip_solver* s, const char* path);
 
 /* ---- reverse mode (adjoint) ----------------------------------------------
  * No counterpart in the reference: the differentiable-solver extension for
- * gradient-based callers (parameter fitting, inverse problems).  Fully
- * periodic domains, CLIP_PREC_F64 or CLIP_PREC_F32 handles.
+ * gradient-based callers (parameter fitting, inverse problems, inflow
+ * control).  CLIP_PREC_F64 or CLIP_PREC_F32 handles; periodic, wall, velocity
+ * inlet, free-convect, Neumann and do-nothing sides (a slip side only
+ * together with a wall side, where it acts as one).
  * clip_adjoint_forward builds the initial state of the case
  * (KernelInitializeDistributions, NsAllen.cu:137-202, with the first
- * collision) from phi0 on the interior nodes (x fastest; ghosts by periodic
- * wrap), runs nsteps steps keeping the state every checkpoint_every steps,
- * and writes phi after the last step (interior, x fastest) to phi_out.
+ * collision) from phi0 on the interior nodes (x fastest; a ghost cell copies
+ * phi0 at its wrapped position when every axis it lies outside of is
+ * periodic, else it keeps the case's initial phase field), runs nsteps steps
+ * keeping the state (populations, phi with its ghost ring, u, outflow planes)
+ * every checkpoint_every steps, and writes phi after the last step (interior,
+ * x fastest) to phi_out.
  * clip_adjoint_backward takes dL/dphi_out and returns dL/dphi0 and, if the
  * pointers are non-NULL, dL/dsigma (through beta, kappa and the initial p)
- * and dL/dmobility; it recomputes each segment between kept states once. */
+ * and dL/dmobility; it recomputes each segment between kept states once
+ * (twice with an inner interval, clip_adjoint_set_nesting). */
 int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out);
 int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* grad_phi0, double* grad_sigma,
                           double* grad_mobility);
@@ -273,6 +279,19 @@ int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* gr
  * derivative ~ 1 / |grad phi| amplifies rounding without bound over long
  * horizons; elsewhere the gradient is unchanged.  c = 0.1 is a good choice. */
 int clip_adjoint_set_normal_floor(clip_solver* s, double normal_floor);
+/* Second checkpoint level of the reverse sweep: 0 < inner_every <
+ * checkpoint_every keeps, while a segment is recomputed, a state every
+ * inner_every steps and recomputes each of those sub-segments once more.
+ * Memory: nsteps / checkpoint_every + checkpoint_every / inner_every +
+ * inner_every states instead of nsteps / checkpoint_every + checkpoint_every.
+ * 0 (default): one level.  Read by the next clip_adjoint_forward. */
+int clip_adjoint_set_nesting(clip_solver* s, int inner_every);
+/* dL/dA_m of the inlet modulation u_in = value (1 + sum_m A_m cos((m+1) theta
+ * + phase_m)) (clip_config inlet_mode_amp) from the last clip_adjoint_backward,
+ * m < n (entries past inlet_modes are 0).  The amplitudes enter through the
+ * inlet ghost populations (JetBoundary, Solver.cu:578-619).  CLIP_ERR_STATE
+ * before a backward sweep. */
+int clip_adjoint_inlet_gradient(clip_solver* s, double* grad, int n);
 void clip_destroy(clip_solver* s);
 const char* clip_last_error(void);
 

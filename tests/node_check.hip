@@ -183,3 +183,78 @@ int init_rev(int dim, const double* phys, double sigma, double half, double c0, 
     return 0;
 }
 }
+
+// ---------------------------------------------------------------- adjoint maps
+// The host-built transposes of the reverse step (adjoint_maps.hpp) on a dense
+// grid, for the CPU tests against the oracle's boundary phases.
+#include "adjoint_maps.hpp"
+
+namespace {
+template <class L>
+int pull_eval(const clip_config* cfg, const double* F, const double* prev, const double* u, double* out) {
+    const DevGrid g = grid_of_config(*cfg, L::D, 1);
+    const DevBC bc = bc_of_config(*cfg, L::D);
+    const auto vf = inlet_flags(*cfg, g, L::D);
+    long long poff[6], tot = 0;
+    for (int s = 0; s < 6; ++s) {
+        poff[s] = tot;
+        if (s < 2 * L::D && cfg->bc_type[s] == CLIP_BC_FREE_CONVECT) tot += plane_size(g, s) * L::Q;
+    }
+    SymPuller<L> sp{g, bc, vf};
+    std::vector<PullTerm> terms;
+    for (long long i = 0; i < g.nodes * L::Q; ++i) out[i] = 0.0;
+    for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
+        for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
+            for (int i = g.dmin[0]; i <= g.dmax[0]; ++i) {
+                const int c[3] = {i, j, k};
+                bool shell = false;
+                for (int a = 0; a < L::D; ++a) shell |= !bc.both_periodic[a] && (c[a] == g.dmin[a] || c[a] == g.dmax[a]);
+                const long long x = g.idx(i, j, k);
+                for (int q = 0; q < L::Q; ++q) {
+                    double v = 0.0;
+                    if (!shell) {
+                        int s3[3] = {i - L::ex(q), j - L::ey(q), k - L::ez(q)};
+                        for (int a = 0; a < L::D; ++a) {
+                            const int span = g.dmax[a] - g.dmin[a] + 1;
+                            if (s3[a] < g.dmin[a]) s3[a] += span;
+                            if (s3[a] > g.dmax[a]) s3[a] -= span;
+                        }
+                        v = F[(long long)q * g.nodes + g.idx(s3[0], s3[1], s3[2])];
+                    } else {
+                        terms.clear();
+                        sp.lvl3(c, q, terms);
+                        for (const auto& t : terms) {
+                            double w = 1.0;
+                            if (t.fc >= 0) {
+                                const double un = std::fabs(u[(long long)(t.fc / 2) * g.nodes + t.nb]);
+                                w = (t.branch == 0 ? un : 1.0) / (1.0 + un);
+                            }
+                            v += w * (t.kind == 1 ? prev[poff[t.side] + t.idx] : F[t.idx]);
+                        }
+                    }
+                    out[(long long)q * g.nodes + x] = v;
+                }
+            }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+// f_post at every interior node from the stored populations F (SoA, dense,
+// ghosts included, inlet ghost values in place), the outflow planes (free-
+// convect sides concatenated, q * plane + t each) and u (SoA): the bulk pull
+// and the boundary chain's terms.
+int maps_pull_eval(const clip_config* cfg, const double* F, const double* prev, const double* u, double* out) {
+    return cfg->dim == 2 ? pull_eval<D2Q9>(cfg, F, prev, u, out) : pull_eval<D3Q19>(cfg, F, prev, u, out);
+}
+// Source of every node of the new phi under one ghost refresh (-2: not written).
+int maps_ring(const clip_config* cfg, long long* src) {
+    const int D = cfg->dim;
+    const DevGrid g = grid_of_config(*cfg, D, 1);
+    const DevBC bc = bc_of_config(*cfg, D);
+    const auto vf = inlet_flags(*cfg, g, D);
+    const std::vector<long long> s = ring_sources(g, bc, D, vf);
+    for (long long i = 0; i < g.nodes; ++i) src[i] = s[i];
+    return 0;
+}
+}

@@ -150,6 +150,72 @@ def test_adjoint_fp32_against_fp64(n, radius, tol_phi, tol_par):
     assert e0 <= tol_phi and es <= tol_par and em <= tol_par
 
 
+def test_c4_workload_against_central_differences():
+    """The C4 workload (256^2 drop, R = 50, density ratio 1000, phi0 with the
+    seeded band perturbation, target from sigma x 1.2 and mobility x 0.8,
+    states kept every 10) at 50 steps, where central differences of the loss
+    converge (the same value for h = 1e-4 .. 1e-7 of the parameter): the
+    adjoint with the normal floor 0.1 matches them (measured 1.1e-3 sigma,
+    4.6e-5 mobility, 1.6e-4 on 16 band entries of dL/dphi0).  The exact
+    derivative (floor 0) is dominated by the normal on rounding-noise
+    plateaus (d n / d grad phi ~ 1 / |grad phi|) and overflows even here.
+    Past ~150 steps the loss has no converged finite-difference derivative:
+    the measured table is in the design notes."""
+    cfg, phi0 = setup(256, 50)
+    T = 50
+    target = target_of(cfg, phi0, T)
+    res = {}
+    for fl in (0.0, 0.1):
+        s = Solver(cfg, "f64")
+        phiT = s.adjoint_forward(phi0, T, 10)
+        res[fl] = s.adjoint_backward(phiT - target, fl)
+    rng = np.random.default_rng(1)
+    band = np.argwhere((phi0 > 0.05) & (phi0 < 0.95))
+    picks = band[rng.choice(len(band), 16, replace=False)].tolist()
+    fd0 = []
+    for (i, j) in picks:
+        h = 1e-6 * abs(phi0[i, j])
+        pp, pm = phi0.copy(), phi0.copy()
+        pp[i, j] += h
+        pm[i, j] -= h
+        fd0.append((loss_of(cfg, pp, target, T) - loss_of(cfg, pm, target, T)) / (2 * h))
+    fd0 = np.array(fd0)
+    h = 1e-6 * cfg.sigma
+    fds = (loss_of(cfg, phi0, target, T, sigma=cfg.sigma + h) - loss_of(cfg, phi0, target, T, sigma=cfg.sigma - h)) / (2 * h)
+    h = 1e-6 * cfg.mobility
+    fdm = (loss_of(cfg, phi0, target, T, mobility=cfg.mobility + h)
+           - loss_of(cfg, phi0, target, T, mobility=cfg.mobility - h)) / (2 * h)
+    g0, gs, gm = res[0.1]
+    e0 = np.linalg.norm(np.array([g0[i, j] for (i, j) in picks]) - fd0) / np.linalg.norm(fd0)
+    es, em = abs(gs - fds) / abs(fds), abs(gm - fdm) / abs(fdm)
+    exact_scale = float(np.max(np.abs(res[0.0][0])))
+    print(f"C4 workload, {T} steps, floor 0.1 vs central differences: sigma {es:.2e} mobility {em:.2e} "
+          f"phi0 (16 band entries) {e0:.2e}; floor 0 max |dL/dphi0| {exact_scale:.2e}", flush=True)
+    assert es <= 3e-3 and em <= 2e-4 and e0 <= 1e-3
+    assert not (exact_scale < 1e3), "the unregularised derivative was expected to blow up on the plateaus"
+
+
+def test_c4_workload_fp32_against_fp64_500_steps():
+    """C4's full horizon (500 steps, kept every 50, floor 0.1): the FP32
+    reverse sweep stays finite and tracks the FP64 one on the parameter
+    gradients (measured sigma 4.7e-2, mobility 4.2e-2; dL/dphi0 0.71 rel-L2,
+    the FP32 trajectory having departed from the FP64 one)."""
+    cfg, phi0 = setup(256, 50)
+    T, K = 500, 50
+    target = target_of(cfg, phi0, T)
+    out = {}
+    for prec in ("f64", "f32"):
+        s = Solver(cfg, prec)
+        phiT = s.adjoint_forward(phi0, T, K)
+        out[prec] = s.adjoint_backward(phiT - target, 0.1)
+        assert np.all(np.isfinite(out[prec][0]))
+    e0 = rel(out["f32"][0], out["f64"][0])
+    es = abs(out["f32"][1] - out["f64"][1]) / abs(out["f64"][1])
+    em = abs(out["f32"][2] - out["f64"][2]) / abs(out["f64"][2])
+    print(f"C4 500 steps fp32 vs fp64: phi0 {e0:.2e} sigma {es:.2e} mobility {em:.2e}", flush=True)
+    assert es <= 0.1 and em <= 0.1 and e0 <= 1.0
+
+
 def test_autograd_function():
     from clip_amd.autograd import coupled_steps
     cfg, phi0 = setup(32, 8)
@@ -168,11 +234,16 @@ def test_autograd_function():
     assert sg.grad.item() == gs and mb.grad.item() == gm
 
 
-def test_adjoint_rejects_non_periodic_and_mixed():
+def test_adjoint_rejects_16bit_storage_and_lone_slip_sides():
+    """The reverse sweep recomputes states exactly from FP64 / FP32 states
+    only; a slip side acts as a wall only next to a wall side (the
+    reference's half bounce-back launch gate), which the reverse does not
+    follow otherwise."""
     from clip_amd import ClipError
-    cfg = Config.from_string(cases.rti2d(16, 32))
-    with pytest.raises(ClipError):
-        Solver(cfg, "f64").adjoint_forward(np.zeros((16, 32)), 2, 1)
     cfg, phi0 = setup(16, 4)
     with pytest.raises(ClipError):
         Solver(cfg, "mixed_f16").adjoint_forward(phi0, 2, 1)
+    slip = Config.from_string(cases.box2d([("x-", "periodic"), ("x+", "periodic"), ("y-", "slip wall"),
+                                           ("y+", "free convect")], n=16, ny=16))
+    with pytest.raises(ClipError):
+        Solver(slip, "f64").adjoint_forward(np.full((16, 16), 0.5), 2, 1)
