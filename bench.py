@@ -63,40 +63,68 @@ def load_traffic(kernel, workload, precision):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(text, steps_cap, seconds_budget):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(text, steps, runs=3):
     """FP64 OpenMP oracle (CLIP's per-node phases in CLIP's AoS layout) on the
-    same configuration: up to `steps_cap` steps after initialisation, each
-    timed on its own; the median step gives the rate."""
+    same configuration: `runs` timed runs of `steps` steady-state steps after
+    initialisation and one untimed step; the median run gives the rate."""
     from clip_amd import Config
     from tests.oracle_lib import Oracle
 
     cfg = Config.from_string(text)
     orc = Oracle(cfg).init()
+    orc.step(1)
     interior = 1
     for a in range(cfg.dim):
         interior *= cfg.extents()[a] - 2
-    times, t_start = [], time.perf_counter()
-    while len(times) < steps_cap:
+    per_step = []
+    for _ in range(runs):
         t0 = time.perf_counter()
-        orc.step(1)
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > seconds_budget:
-            break
-    dt = sorted(times)[len(times) // 2]
+        orc.step(steps)
+        per_step.append((time.perf_counter() - t0) / steps)
+    dt = sorted(per_step)[len(per_step) // 2]
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     del orc
     return {"value": interior / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": f"median of {len(times)} timed step(s) of the same workload after initialisation, FP64 "
-                      f"oracle, OpenMP over {cores} threads (OMP_PROC_BIND=close, OMP_PLACES=cores), "
-                      f"{sum(times):.1f} s"}
+            "cpu_model": cpu_model(),
+            "sample": f"median of {runs} runs of {steps} steady-state steps of the same workload (after "
+                      f"initialisation and one untimed step), FP64 oracle, OpenMP over {cores} threads "
+                      f"(OMP_PROC_BIND=close, OMP_PLACES=cores) on {cpu_model()}, "
+                      f"{sum(per_step) * steps:.1f} s timed"}
 
 
-def measure(workload, precision, steps=50, warmup=5):
-    """MLUPS and the k_fused roofline of one more workload/precision."""
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    import numpy as np
+    den = float(np.linalg.norm(b.ravel()))
+    return float(np.linalg.norm((a - b).ravel())) / den if den > 0 else float(np.linalg.norm(a.ravel()))
+
+
+def measure(workload, precision, steps=50, warmup=5, graph=False, against_f64=False):
+    """MLUPS and the k_fused roofline of one more workload/precision
+    (graph: steps through the captured two-step HIP graph; against_f64: the
+    rel-L2 of phi, u, p and the phi-sum drift against an FP64 run of the
+    same number of steps)."""
+    progress(f"{workload} {precision}{' (HIP graph)' if graph else ''}")
     import torch
     from clip_amd import Config, Solver
     cfg = Config.from_string(workload_text(workload))
     sol = Solver(cfg, precision).init_fresh()
+    if graph:
+        sol.enable_graph(True)
     inf = sol.info()
     sol.step(warmup)
     sol.synchronize()
@@ -107,16 +135,34 @@ def measure(workload, precision, steps=50, warmup=5):
     sol.enable_timing(True)
     sol.step(min(steps, 20))
     tm = sol.last_timing()
-    sol.close()
+    sol.enable_timing(False)
+    total_steps = warmup + steps + min(steps, 20)
+    errors = None
+    if against_f64 and precision != "f64":
+        fields = {f: sol.interior(f) for f in ("phi", "u", "p")}
+        s0 = sol.sum("phi")
+        sol.close()
+        ref = Solver(cfg, "f64").init_fresh()
+        r0 = ref.sum("phi")
+        ref.step(total_steps)
+        errors = {f"rel_l2_{f}": rel_l2(fields[f], ref.interior(f)) for f in fields}
+        errors["phi_sum_drift"] = abs(s0 - ref.sum("phi")) / abs(r0)
+        errors["steps"] = total_steps
+        ref.close()
+    else:
+        sol.close()
     torch.cuda.synchronize()
     n = int(inf.interior)
     k = min(steps, 20)
     fused_ms = tm["fused_ms"] / k
     alg = n * (4 * inf.q * pop_bytes(precision) + (2 * inf.dim + 1) * mac_bytes(precision))
-    return {"workload": workload, "precision": precision, "interior_nodes": n, "lattice": f"D{inf.dim}Q{inf.q}",
-            "MLUPS": round(n * steps / dt / 1e6, 2), "ms_per_step": round(dt * 1e3 / steps, 4),
-            "kernel_ms_per_step": {kk: round(v / k, 4) for kk, v in tm.items()},
-            "k_fused_hbm_frac": round(alg / (fused_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"workload": workload, "precision": precision, "interior_nodes": n, "lattice": f"D{inf.dim}Q{inf.q}",
+           "hip_graph": graph, "MLUPS": round(n * steps / dt / 1e6, 2), "ms_per_step": round(dt * 1e3 / steps, 4),
+           "kernel_ms_per_step": {kk: round(v / k, 4) for kk, v in tm.items()},
+           "k_fused_hbm_frac": round(alg / (fused_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if errors:
+        out["against_f64"] = errors
+    return out
 
 
 def copy_bandwidth(mib=2048, reps=10):
@@ -148,6 +194,7 @@ def adjoint_measure(precision, n=256, radius=50, steps=500, every=50):
     NSAllen::solve), the forward with states kept every `every` steps, and the
     reverse sweep (which recomputes each segment once), with the interface
     normal's derivative bounded on plateaus (normal floor 0.1)."""
+    progress(f"drop2d{n} adjoint {precision}")
     import numpy as np
     import torch
     from clip_amd import Config, Solver
@@ -179,11 +226,84 @@ def adjoint_measure(precision, n=256, radius=50, steps=500, every=50):
     g0, gs, gm = s.adjoint_backward(phiT - target, 0.1)
     t2 = time.perf_counter()
     s.close()
+    # the same workload over 50 steps, where central differences of the loss
+    # converge: the floored adjoint against them (past ~150 steps the loss has
+    # no converged finite-difference derivative at this density ratio)
+    T = 50
+    tgt50 = Solver(cfg.with_params(sigma=1.2 * cfg.sigma, mobility=0.8 * cfg.mobility),
+                   "f64").adjoint_forward(phi0, T, T)
+
+    def loss(sig=None, mob=None):
+        q = Solver(cfg.with_params(sigma=sig, mobility=mob), "f64").adjoint_forward(phi0, T, T)
+        return 0.5 * float(np.sum((q - tgt50) ** 2))
+    hs, hm = 1e-6 * cfg.sigma, 1e-6 * cfg.mobility
+    fds = (loss(sig=cfg.sigma + hs) - loss(sig=cfg.sigma - hs)) / (2 * hs)
+    fdm = (loss(mob=cfg.mobility + hm) - loss(mob=cfg.mobility - hm)) / (2 * hm)
+    s50 = Solver(cfg, precision)
+    p50 = s50.adjoint_forward(phi0, T, 10)
+    _, gs50, gm50 = s50.adjoint_backward(p50 - tgt50, 0.1)
+    s50.close()
     return {"workload": f"drop2d{n}_adjoint", "precision": precision, "lattice": "D2Q9", "steps": steps,
             "checkpoint_every": every, "normal_floor": 0.1, "forward_ms": round(fwd * 1e3, 3),
             "forward_with_checkpoints_ms": round((t1 - t0) * 1e3, 3), "backward_ms": round((t2 - t1) * 1e3, 3),
             "adjoint_over_forward": round((t2 - t1) / fwd, 3),
             "loss": float(0.5 * np.sum((phiT - target) ** 2)), "grad_sigma": gs, "grad_mobility": gm,
+            "grad_phi0_norm": float(np.linalg.norm(g0)),
+            "fd_check_50_steps": {"normal_floor": 0.1, "rel_err_sigma": abs(gs50 - fds) / abs(fds),
+                                  "rel_err_mobility": abs(gm50 - fdm) / abs(fdm),
+                                  "central_difference_h": "1e-6 x parameter, FP64 forward"}}
+
+
+def adjoint_c5_measure(precision="f32", steps=200, every=20, inner=5):
+    """Reverse mode on the 3D jet (D3Q19, 192x192x768, modulated disc inlet,
+    outflow): loss = 1/2 |phi_T - phi_target|^2 after `steps` steps from the
+    case's initial phase field, target from a run with the 8 inlet mode
+    amplitudes x 1.5; gradients w.r.t. the 8 amplitudes, sigma and the
+    mobility; states kept every `every` steps and, inside each recomputed
+    segment, every `inner` steps (two levels).  Peak device memory from
+    hipMemGetInfo (through torch) after the checkpointed forward."""
+    progress(f"jet3d192x192x768 adjoint {precision}")
+    import numpy as np
+    import torch
+    from clip_amd import Config, Solver
+    text = workload_text("jet3d192x192x768")
+    cfg = Config.from_string(text)
+    tcfg = Config.from_string(text)
+    for m in range(tcfg.raw.inlet_modes):
+        tcfg.raw.inlet_mode_amp[m] *= 1.5
+    tgt = Solver(tcfg, precision).init_fresh()
+    tgt.step(steps)
+    target = tgt.interior("phi")[..., 0]
+    tgt.close()
+    base = Solver(cfg, precision).init_fresh()
+    phi0 = base.interior("phi")[..., 0]
+    base.step(5)
+    base.synchronize()
+    t0 = time.perf_counter()
+    base.step(steps)
+    base.synchronize()
+    fwd = time.perf_counter() - t0
+    base.close()
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    s = Solver(cfg, precision)
+    t0 = time.perf_counter()
+    phiT = s.adjoint_forward(phi0, steps, every, inner_every=inner)
+    t1 = time.perf_counter()
+    free1, _ = torch.cuda.mem_get_info()
+    g0, gs, gm = s.adjoint_backward(phiT - target, 0.1)
+    t2 = time.perf_counter()
+    ga = s.adjoint_inlet_gradient()
+    held = int(s.info().device_bytes)
+    s.close()
+    return {"workload": "jet3d192x192x768_adjoint", "precision": precision, "lattice": "D3Q19", "steps": steps,
+            "checkpoint_every": every, "inner_checkpoint_every": inner, "normal_floor": 0.1,
+            "forward_ms": round(fwd * 1e3, 3), "forward_with_checkpoints_ms": round((t1 - t0) * 1e3, 3),
+            "backward_ms": round((t2 - t1) * 1e3, 3), "adjoint_over_forward": round((t2 - t1) / fwd, 3),
+            "peak_device_bytes_used": int(total - free1), "device_bytes_before": int(total - free0),
+            "handle_device_bytes": held,
+            "loss": float(0.5 * np.sum((phiT - target) ** 2)),
+            "grad_inlet_amplitudes": [float(x) for x in ga], "grad_sigma": gs, "grad_mobility": gm,
             "grad_phi0_norm": float(np.linalg.norm(g0))}
 
 
@@ -195,8 +315,7 @@ def main():
     ap.add_argument("--workload", default="bubble3d256")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "mixed_f16", "mixed_bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-steps", type=int, default=8, help="steady-state steps per CPU baseline run (3 runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode measurement")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and jet lines")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -204,6 +323,8 @@ def main():
                          "its rocprofv3 FETCH_SIZE / WRITE_SIZE calibrate the traffic counters")
     ap.add_argument("--adjoint-only", choices=["f32", "f64"], default=None,
                     help="only run the reverse-mode measurement at this precision (for kernel traces)")
+    ap.add_argument("--adjoint-c5-only", choices=["f32", "f64"], default=None,
+                    help="only run the 3D jet reverse-mode measurement at this precision")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -227,6 +348,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_measure(args.adjoint_only)), flush=True)
+        return
+    if args.adjoint_c5_only:
+        print(json.dumps(adjoint_c5_measure(args.adjoint_c5_only)), flush=True)
         return
     text = workload_text(args.workload)
     cfg = Config.from_string(text)
@@ -290,12 +414,18 @@ def main():
     }
     if not args.no_extra:
         other = "mixed_f16" if args.precision == "f64" else "f64"
-        line["extra"] = [measure(args.workload, other), measure("jet3d192x192x768", "mixed_f16"),
-                         measure("jet3d192x192x768", "f64")]
+        line["extra"] = [measure(args.workload, other, against_f64=True),
+                         measure("jet3d192x192x768", "mixed_f16", against_f64=True),
+                         measure("jet3d192x192x768", "f64"),
+                         measure("drop2d128", "f64", steps=1000, warmup=20),
+                         measure("drop2d128", "f64", steps=1000, warmup=20, graph=True),
+                         measure("rti2d", "f64", steps=1000, warmup=20, graph=True),
+                         measure("rti2d", "f32", steps=1000, warmup=20, graph=True)]
     if not args.no_adjoint:
-        line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64")]
+        line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64"), adjoint_c5_measure("f32")]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(text, args.cpu_steps, args.cpu_seconds)
+        progress("CPU baseline (FP64 OpenMP oracle)")
+        line["cpu_baseline"] = cpu_baseline(text, args.cpu_steps)
     print(json.dumps(line), flush=True)
 
 

@@ -237,6 +237,11 @@ class Solver:
     def checkpoint_load(self, directory):
         _abi.check(self._lib.clip_checkpoint_load(self._h, str(directory).encode()))
 
+    def enable_graph(self, on=True):
+        """Step through a captured HIP graph of two steps (clip_enable_graph)."""
+        _abi.check(self._lib.clip_enable_graph(self._h, 1 if on else 0))
+        return self
+
     def enable_timing(self, on=True):
         _abi.check(self._lib.clip_enable_timing(self._h, 1 if on else 0))
 

@@ -120,6 +120,7 @@ SIGNATURES = {
     "clip_get_info": (ctypes.c_int, [_P, ctypes.POINTER(ClipInfo)]),
     "clip_last_step_timing": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "clip_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "clip_enable_graph": (ctypes.c_int, [_P, ctypes.c_int]),
     "clip_adjoint_forward": (ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int, _D]),
     "clip_adjoint_backward": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "clip_adjoint_set_normal_floor": (ctypes.c_int, [_P, ctypes.c_double]),

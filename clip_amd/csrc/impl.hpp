@@ -76,6 +76,7 @@ struct Impl {
     virtual void set_stream(hipStream_t s) = 0;
     virtual void timing(double* total, double* ghost, double* phase, double* fused) const = 0;
     virtual void enable_timing(bool on) = 0;
+    virtual void enable_graph(bool on) = 0;
     // Reverse mode from an interior phi0 (x fastest) over nsteps steps.
     virtual void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every,
                                  std::vector<double>& phi_out) = 0;

@@ -270,6 +270,12 @@ int clip_last_step_timing(const clip_solver* s, double* total, double* ghost, do
     return CLIP_OK;
 }
 
+int clip_enable_graph(clip_solver* s, int on) {
+    if (!s) return set_error(CLIP_ERR_ARG, "null handle");
+    s->impl->enable_graph(on != 0);
+    return CLIP_OK;
+}
+
 int clip_enable_timing(clip_solver* s, int on) {
     if (!s) return set_error(CLIP_ERR_ARG, "null handle");
     s->impl->enable_timing(on != 0);

@@ -212,6 +212,7 @@ struct SolverImpl : Impl {
 
     ~SolverImpl() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : graphs) (void)hipGraphExecDestroy(e.second);
         adj_free();
         for (void* ptr : allocations) (void)hipFree(ptr);
         for (auto& e : ev)
@@ -449,39 +450,89 @@ struct SolverImpl : Impl {
         initialized = true;
     }
 
+    // One step's launches from the current buffer indices, then every index
+    // flips (the collision just written is pulled next).
+    void one_step() {
+        CT* phi_old = phib[phc];
+        CT* phi_new = phib[1 - phc];
+        StepArgs<ST, CT> a = args(cur, ucur, pcur, phi_new, phi_old);
+        const bool tm = timing_on && !capturing;
+        if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
+        ghost_refresh(phi_old, phi_new);
+        if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
+        launch_phase_pull(a);
+        if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
+        launch_fused_pull<MODE_FULL>(a);
+        if (tm) {
+            HIP_CHECK(hipEventRecord(ev[3], stream));
+            HIP_CHECK(hipEventSynchronize(ev[3]));
+            float m1, m2, m3;
+            HIP_CHECK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+            HIP_CHECK(hipEventElapsedTime(&m2, ev[1], ev[2]));
+            HIP_CHECK(hipEventElapsedTime(&m3, ev[2], ev[3]));
+            t_ghost += m1;
+            t_phase += m2;
+            t_fused += m3;
+            t_total += m1 + m2 + m3;
+        }
+        cur = 1 - cur;
+        ucur = 1 - ucur;
+        pcur = 1 - pcur;
+        phc = 1 - phc;
+        direct_old = false;
+        old_ref_zero = false;
+        step_count++;
+    }
+
+    // HIP graph of two steps: every index flips twice, so the graph of one
+    // index state replays as many times as needed.  Its nodes are kernel
+    // launches only (no memset / copy nodes).
+    bool graph_on = false, capturing = false;
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;
+    int index_key() const { return cur | (ucur << 1) | (pcur << 2) | (phc << 3); }
+    hipGraphExec_t pair_graph() {
+        const int key = index_key();
+        for (auto& e : graphs)
+            if (e.first == key) return e.second;
+        const int c0 = cur, u0 = ucur, p0 = pcur, f0 = phc;
+        const bool d0 = direct_old, z0 = old_ref_zero;
+        const long long s0 = step_count;
+        hipGraph_t gph = nullptr;
+        capturing = true;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        one_step();
+        one_step();
+        const hipError_t ec = hipStreamEndCapture(stream, &gph);
+        capturing = false;
+        HIP_CHECK(ec);
+        hipGraphExec_t exec = nullptr;
+        HIP_CHECK(hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(gph));
+        // capture records the launches without running them
+        cur = c0;
+        ucur = u0;
+        pcur = p0;
+        phc = f0;
+        direct_old = d0;
+        old_ref_zero = z0;
+        step_count = s0;
+        graphs.push_back({key, exec});
+        return exec;
+    }
+    void enable_graph(bool on) override { graph_on = on; }
+
     void step(int n) override {
         t_total = t_ghost = t_phase = t_fused = 0;
-        for (int s = 0; s < n; ++s) {
-            CT* phi_old = phib[phc];
-            CT* phi_new = phib[1 - phc];
-            StepArgs<ST, CT> a = args(cur, ucur, pcur, phi_new, phi_old);
-            const bool tm = timing_on;
-            if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
-            ghost_refresh(phi_old, phi_new);
-            if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
-            launch_phase_pull(a);
-            if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
-            launch_fused_pull<MODE_FULL>(a);
-            if (tm) {
-                HIP_CHECK(hipEventRecord(ev[3], stream));
-                HIP_CHECK(hipEventSynchronize(ev[3]));
-                float m1, m2, m3;
-                HIP_CHECK(hipEventElapsedTime(&m1, ev[0], ev[1]));
-                HIP_CHECK(hipEventElapsedTime(&m2, ev[1], ev[2]));
-                HIP_CHECK(hipEventElapsedTime(&m3, ev[2], ev[3]));
-                t_ghost += m1;
-                t_phase += m2;
-                t_fused += m3;
-                t_total += m1 + m2 + m3;
+        if (graph_on && !timing_on) {
+            while (n >= 2) {
+                HIP_CHECK(hipGraphLaunch(pair_graph(), stream));
+                step_count += 2;
+                direct_old = false;
+                old_ref_zero = false;
+                n -= 2;
             }
-            cur = 1 - cur;
-            ucur = 1 - ucur;
-            pcur = 1 - pcur;
-            phc = 1 - phc;
-            direct_old = false;
-            old_ref_zero = false;
-            step_count++;
         }
+        for (int s = 0; s < n; ++s) one_step();
     }
 
     void timing(double* total, double* gh, double* pha, double* fu) const override {

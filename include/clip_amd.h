@@ -245,6 +245,12 @@ int clip_get_info(const clip_solver* s, clip_info* out);
 int clip_last_step_timing(const clip_solver* s, double* total_ms, double* ghost_ms,
                           double* phase_ms, double* fused_ms);
 int clip_enable_timing(clip_solver* s, int on);
+/* Step through a HIP graph (no counterpart in the reference): clip_step then
+ * replays a captured graph of two steps (the ping-pong period of the buffers)
+ * per pair, cutting the host launch cost of small grids where a step is a
+ * few microseconds per kernel.  Results are identical to plain stepping.
+ * Ignored while per-kernel timing is on. */
+int clip_enable_graph(clip_solver* s, int on);
 /* VTSwriter::writeVTSBinaryFile (src/io/VTSwriter.cu:84-205): write the
  * current Velocity, C (phi), P and Rho of the interior nodes to one ASCII VTK
  * StructuredGrid file at `path`.  Fields leave the device only on this call. */
