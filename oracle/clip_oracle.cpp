@@ -279,6 +279,49 @@ inline double gamma_u(const Lattice& L, int q, const double* u) {
 // by s, back-transform with M^-1), evaluated as the identity plus its
 // stress-mode part: gneq + (s - 1) M^-1[:, stress] m[stress].  Same linear map
 // (M^-1 M = I); only the stress rows of M and columns of M^-1 are touched.
+#ifdef CLIP_ORACLE_LITERAL
+// Built with -DCLIP_ORACLE_LITERAL (tests/test_oracle_order.py): the
+// reference's literal operation order instead, m = M gneq over all rows (each
+// row summed over q ascending, WMRT.cuh:54-126), the stress moments scaled by
+// s (NsAllen.cu:428-429 / 448-452), then gneq = M^-1 m with M^-1 as integer
+// numerators over the reference's per-row denominators (D2Q9: 36 for every
+// row; D3Q19: 6, 36, 36, 72 x 4, 144 x 12; WMRT.cuh:77-92, 134-184), each row
+// summed over the moments ascending and divided once.
+void wmrt_filter(const Lattice& L, double* v, double s) {
+    static const double den2[9] = {36, 36, 36, 36, 36, 36, 36, 36, 36};
+    static const double den3[19] = {6, 36, 36, 72, 72, 72, 72, 144, 144, 144, 144, 144, 144, 144, 144, 144, 144, 144, 144};
+    const double* den = (L.Q == 9) ? den2 : den3;
+    auto accumulate = [](double acc, bool first, double c, double x) {
+        if (first) return c == 1.0 ? x : (c == -1.0 ? -x : c * x);
+        if (c == 1.0) return acc + x;
+        if (c == -1.0) return acc - x;
+        return acc + c * x;
+    };
+    double m[19];
+    for (int r = 0; r < L.Q; ++r) {
+        double acc = 0;
+        bool first = true;
+        for (int q = 0; q < L.Q; ++q) {
+            if (L.M[r][q] == 0.0) continue;
+            acc = accumulate(acc, first, L.M[r][q], v[q]);
+            first = false;
+        }
+        m[r] = acc;
+    }
+    for (int k = 0; k < L.nstress; ++k) m[L.stress[k]] = m[L.stress[k]] * s;
+    for (int q = 0; q < L.Q; ++q) {
+        double acc = 0;
+        bool first = true;
+        for (int r = 0; r < L.Q; ++r) {
+            const double c = std::nearbyint(L.Minv[q][r] * den[q]);
+            if (c == 0.0) continue;
+            acc = accumulate(acc, first, c, m[r]);
+            first = false;
+        }
+        v[q] = acc / den[q];
+    }
+}
+#else
 void wmrt_filter(const Lattice& L, double* v, double s) {
     double m[5];
     for (int k = 0; k < L.nstress; ++k) {
@@ -293,6 +336,7 @@ void wmrt_filter(const Lattice& L, double* v, double s) {
         v[q] = v[q] + sm1 * (acc / L.Minv_den[q]);
     }
 }
+#endif
 
 // tau(phi) with the clamp of NsAllen.cu:418-426.
 inline double tau_of(const clip_config& c, double phi) {
