@@ -285,6 +285,41 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a, CT* lall) {
     lall[y] = total;
 }
 
+// Fully periodic domains: the ring of phi' is the periodic wrap of phi^n's
+// interior, so the stencil transpose gathers with wrapped reads in one
+// kernel at interior nodes: lphin = the adjoint of phi' and, with LAG, the
+// reads that hit the lagged ring go to lPhi (the adjoint of phi^n); without
+// LAG (the initial state, whose ring wraps phi0) they stay in lphin.
+template <class L, class CT, bool LAG>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_phi_periodic(AdjArgs<CT> a) {
+    constexpr int D = L::D;
+    const DevGrid& g = a.g;
+    int c[3];
+    if (!interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+    const long long n = g.idx(c[0], c[1], c[2]);
+    CT total = a.lPhi[n] + a.lst[n];
+    CT lag = CT(0);
+    const int zlo = (D == 3) ? -1 : 0, zhi = (D == 3) ? 1 : 0;
+    for (int dz = zlo; dz <= zhi; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                CT wg[3], wl;
+                stencil_weights<D, CT>(dx, dy, dz, wg, wl);
+                const int z[3] = {c[0] - dx, c[1] - dy, c[2] - dz};
+                const bool zin = g.interior(z, D);
+                const int zx = wrap_axis(g, 0, z[0]), zy = wrap_axis(g, 1, z[1]);
+                const int zz = (D == 3) ? wrap_axis(g, 2, z[2]) : z[2];
+                const long long m = g.idx(zx, zy, zz);
+                CT v = wl * a.lst[(long long)(1 + D) * g.nodes + m];
+#pragma unroll
+                for (int d = 0; d < D; ++d) v += wg[d] * a.lst[(long long)(1 + d) * g.nodes + m];
+                if (zin || !LAG) total += v;
+                else lag += v;
+            }
+    a.lphin[n] = total;
+    if (LAG) a.lPhi[n] = lag;
+}
+
 // Transpose of a copy map (host-built rows: target = a cell of the source
 // field, entries' rd = the cells copied from it): out[target] = sum over the
 // row of in[rd].  Targets of no row are left as they are.

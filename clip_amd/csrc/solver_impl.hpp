@@ -1420,10 +1420,16 @@ struct SolverImpl : Impl {
         }
         k_adj_node<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_node");
-        k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
-        launch_check("k_adj_phi");
-        fill(lPhi, g.nodes);
-        launch_rows(rows_ring, [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_ring, lphin, lPhi); });
+        if (!shell_nodes()) {
+            k_adj_phi_periodic<L, CT, true><<<blocks, ADJ_TB, 0, stream>>>(A);
+            launch_check("k_adj_phi");
+        } else {
+            k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
+            launch_check("k_adj_phi");
+            fill(lPhi, g.nodes);
+            launch_rows(rows_ring,
+                        [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_ring, lphin, lPhi); });
+        }
         k_adj_pull<L, CT><<<blocks + 1, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
         launch_check("k_adj_pull");
         launch_rows(rows_pop, [&](unsigned b) { k_adj_csr_pop<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_pop); });
@@ -1513,13 +1519,21 @@ struct SolverImpl : Impl {
         launch_check("k_adj_init");
         k_adj_accumulate<CT><<<1, ADJ_TB, 0, stream>>>(adj_part, (int)blocks, adj_acc);
         launch_check("k_adj_accumulate");
-        k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
-        launch_check("k_adj_phi");
-        fill(lPhi, g.nodes);
-        launch_rows(rows_init, [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_init, lphin, lPhi); });
         std::vector<double> la, lr;
-        download(lphin, (size_t)g.nodes, la);
-        download(lPhi, (size_t)g.nodes, lr);
+        if (!shell_nodes()) {
+            k_adj_phi_periodic<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A);
+            launch_check("k_adj_phi");
+            download(lphin, (size_t)g.nodes, la);
+            lr.assign((size_t)g.nodes, 0.0);
+        } else {
+            k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
+            launch_check("k_adj_phi");
+            fill(lPhi, g.nodes);
+            launch_rows(rows_init,
+                        [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_init, lphin, lPhi); });
+            download(lphin, (size_t)g.nodes, la);
+            download(lPhi, (size_t)g.nodes, lr);
+        }
         gphi0.assign((size_t)interior_count(), 0.0);
         for_interior([&](long long t, long long n) { gphi0[t] = la[n] + lr[n]; });
         double acc[4];
