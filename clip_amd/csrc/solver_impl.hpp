@@ -1070,7 +1070,12 @@ struct SolverImpl : Impl {
     long long poff[6] = {};
     long long prev_elems = 0;           // entries of one plane set over all free-convect sides
 
-    size_t state_elems() const { return (size_t)g.nodes * (2 * QL + 1 + L::D) + 2 * (size_t)prev_elems; }
+    // a state rounded up to whole 128-byte lines, so that every part of every
+    // pool slot keeps the row alignment of alloc_grid
+    size_t state_elems() const {
+        const size_t line = 128 / sizeof(CT);
+        return ((size_t)g.nodes * (2 * QL + 1 + L::D) + 2 * (size_t)prev_elems + line - 1) / line * line;
+    }
     // pool: kept states 0, K, 2K, ... | the final state | inner-level states | leaf states
     CT* slot(size_t k) { return adj_pool + k * state_elems(); }
     CT* snap_ptr(int k) { return slot((size_t)k); }
@@ -1142,6 +1147,12 @@ struct SolverImpl : Impl {
         adj_bytes += count * sizeof(T);
         bytes += count * sizeof(T);
         return (T*)ptr;
+    }
+    // Node array of the reverse sweep with the row alignment of alloc_grid.
+    template <class T>
+    T* adj_alloc_grid(size_t count) {
+        const size_t lead = row_align ? 128 / sizeof(T) - 1 : 0;
+        return adj_alloc<T>(count + lead + 1) + lead;
     }
     void adj_free() {
         for (void* p2 : adj_allocs) (void)hipFree(p2);
@@ -1319,17 +1330,17 @@ struct SolverImpl : Impl {
         const size_t need = pool_states();
         if (need > adj_capacity || !adj_pool) {
             adj_free();
-            adj_pool = adj_alloc<CT>(need * state_elems());
+            adj_pool = adj_alloc_grid<CT>(need * state_elems());
             adj_capacity = need;
             const size_t nq = (size_t)g.nodes * QL;
-            lF = adj_alloc<CT>(nq);
-            lG = adj_alloc<CT>(nq);
-            lfp = adj_alloc<CT>(nq);
-            lgp = adj_alloc<CT>(nq);
-            lU = adj_alloc<CT>((size_t)g.nodes * L::D);
-            lPhi = adj_alloc<CT>((size_t)g.nodes);
-            lphin = adj_alloc<CT>((size_t)g.nodes);
-            lst = adj_alloc<CT>((size_t)g.nodes * (L::D + 2));
+            lF = adj_alloc_grid<CT>(nq);
+            lG = adj_alloc_grid<CT>(nq);
+            lfp = adj_alloc_grid<CT>(nq);
+            lgp = adj_alloc_grid<CT>(nq);
+            lU = adj_alloc_grid<CT>((size_t)g.nodes * L::D);
+            lPhi = adj_alloc_grid<CT>((size_t)g.nodes);
+            lphin = adj_alloc_grid<CT>((size_t)g.nodes);
+            lst = adj_alloc_grid<CT>((size_t)g.nodes * (L::D + 2));
             if (prev_elems) {
                 lP[0] = adj_alloc<CT>(2 * (size_t)prev_elems);
                 lP[1] = adj_alloc<CT>(2 * (size_t)prev_elems);
