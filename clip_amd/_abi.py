@@ -78,6 +78,7 @@ class ClipConfig(ctypes.Structure):
         ("inlet_mode_phase", ctypes.c_double * MAX_INLET_MODES),
         ("has_mobility_override", ctypes.c_int),
         ("mobility_override", ctypes.c_double),
+        ("slip_specular", ctypes.c_int),
     ]
 
 

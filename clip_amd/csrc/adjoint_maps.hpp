@@ -115,17 +115,10 @@ struct SymPuller {
     }
 
     void lvl1(const int c[3], int q, int fc, long long nb, int branch, std::vector<PullTerm>& out) const {
-        if (bc.is_wall) {
-            for (int side = 0; side < 2 * L::D; ++side) {
-                const int a = side >> 1;
-                const int inward = (side & 1) ? -1 : 1;
-                const int t = bc.type[side];
-                if ((t == BC_WALL || t == BC_SLIP_WALL) && L::e(q, a) == inward &&
-                    c[a] == ((side & 1) ? g.dmax[a] : g.dmin[a])) {
-                    load(L::opp(q), c, fc, nb, branch, out);
-                    return;
-                }
-            }
+        const int src = wall_source<L>(g, bc, c, q);
+        if (src >= 0) {
+            load(src, c, fc, nb, branch, out);
+            return;
         }
         raw(c, q, fc, nb, branch, out);
     }

@@ -317,6 +317,14 @@ void parse(const std::vector<std::string>& lines, int dim_override, clip_config*
         kv.array("inletModePhase", cfg->inlet_mode_phase, CLIP_MAX_INLET_MODES);
     }
 
+    // Slip-side behaviour (extension): the reference's (default) or specular.
+    std::string slip;
+    if (kv.word("slipWall", slip)) {
+        const std::string l = lower(slip);
+        if (l == "specular") cfg->slip_specular = 1;
+        else if (l != "reference") throw ConfigError("slipWall must be 'reference' or 'specular'");
+    }
+
     // Boundary blocks.
     for (int s = 0; s < CLIP_NUM_SIDES; ++s) cfg->bc_type[s] = CLIP_BC_UNKNOWN;
     auto bblocks = read_blocks(lines, "boundary");

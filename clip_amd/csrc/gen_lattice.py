@@ -25,6 +25,15 @@ D3Q19_E = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (
            (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1), (0, 1, -1), (0, -1, 1)]
 D3Q19_W = [Fr(1, 3)] + [Fr(1, 18)] * 6 + [Fr(1, 36)] * 12
 
+# In-directions of the minus side of each axis, in the order of the
+# reference's boundary maps: WMRT::wallBCMap (WMRT.cuh:214-237) and
+# WMRT::slipWallBCMap (WMRT.cuh:243-264) list the same directions in the same
+# order for X-, Y- and Z-.  Entry m of a plus side's list is the opposite of
+# the minus side's entry m in the wall map and its mirror image along the
+# axis in the slip map.
+D2Q9_IN_MINUS = [[1, 5, 8], [2, 5, 6]]
+D3Q19_IN_MINUS = [[1, 9, 7, 13, 11], [3, 10, 7, 17, 15], [5, 14, 11, 18, 15]]
+
 
 def d2q9_rows():
     def e2(x, y):
@@ -92,7 +101,22 @@ def lcm(a, b):
     return a * b // gcd(a, b)
 
 
-def emit(name, E, W, rows, stress):
+def mirror(E, q, a):
+    return E.index(tuple(-c if d == a else c for d, c in enumerate(E[q])))
+
+
+def side_lists(E, in_minus):
+    """Per side (X-, X+, Y-, Y+[, Z-, Z+]): the wall map's and the slip map's
+    direction lists."""
+    opp = [E.index(tuple(-c for c in e)) for e in E]
+    wall, slip = [], []
+    for a, lst in enumerate(in_minus):
+        wall += [list(lst), [opp[q] for q in lst]]
+        slip += [list(lst), [mirror(E, q, a) for q in lst]]
+    return wall, slip
+
+
+def emit(name, E, W, rows, stress, in_minus):
     Q = len(E)
     D = len(E[0])
     M = build(rows, E)
@@ -127,10 +151,24 @@ def emit(name, E, W, rows, stress):
     # opposite direction
     opp = [E.index(tuple(-c for c in e)) for e in E]
     out.append(f"static constexpr int {name}_OPP[{Q}] = {{{', '.join(map(str, opp))}}};")
+    # direction mirrored along each axis (identity along axes beyond D)
+    mir = [[mirror(E, q, a) if a < D else q for q in range(Q)] for a in range(3)]
+    out.append(f"static constexpr int {name}_MIRROR[3][{Q}] = {{")
+    for r in mir:
+        out.append("    {" + ", ".join(map(str, r)) + "},")
+    out.append("};")
+    # position of each direction in each side's wall / slip list (-1: absent)
+    wall, slip = side_lists(E, in_minus)
+    for tag, lists in (("WALL_POS", wall), ("SLIP_POS", slip)):
+        out.append(f"static constexpr int {name}_{tag}[6][{Q}] = {{")
+        for sd in range(6):
+            lst = lists[sd] if sd < len(lists) else []
+            out.append("    {" + ", ".join(str(lst.index(q) if q in lst else -1) for q in range(Q)) + "},")
+        out.append("};")
     return "\n".join(out)
 
 
 if __name__ == "__main__":
     print("// lattice_tables.inc — generated by clip_amd/csrc/gen_lattice.py")
-    print(emit("D2Q9", D2Q9_E, D2Q9_W, d2q9_rows(), [7, 8]))
-    print(emit("D3Q19", D3Q19_E, D3Q19_W, d3q19_rows(), [4, 5, 6, 7, 8]))
+    print(emit("D2Q9", D2Q9_E, D2Q9_W, d2q9_rows(), [7, 8], D2Q9_IN_MINUS))
+    print(emit("D3Q19", D3Q19_E, D3Q19_W, d3q19_rows(), [4, 5, 6, 7, 8], D3Q19_IN_MINUS))

@@ -75,6 +75,8 @@ struct DevBC {
     int periodic[3];       // axis wraps (either side periodic: Solver.cu:96, 110, 125)
     int both_periodic[3];  // both sides periodic: no boundary-layer fix-ups on this axis
     int is_wall, is_fc, is_neumann, is_velocity, mirror_gate;
+    int slip_specular;  // slip sides reflect specularly (opt-in; see wall_source)
+    int wall_pass;      // the half-way wall pass runs: any wall side, or a specular slip side
     const unsigned char* vflag[6];  // per velocity side: 1 = ghost node inside the inlet disc
 };
 
@@ -125,7 +127,43 @@ inline DevBC bc_of_config(const clip_config& c, int D) {
     bc.is_neumann = isNeu;
     bc.is_velocity = isVel;
     bc.mirror_gate = isWall || isFC || isSlip || isNeu;
+    bc.slip_specular = c.slip_specular != 0;
+    bc.wall_pass = isWall || (bc.slip_specular && isSlip);
     return bc;
+}
+
+// Half-way wall pass (kernelHalfBounceBack, Solver.cu:248-333) at interior
+// node c for incoming direction q: the direction of the same node's
+// post-collision population that becomes post-streaming population q, or -1
+// when no wall-type side of c takes q in.
+//  - Reference rule (slip_specular = 0; the pass runs when any side is a
+//    wall): wall and slip sides both bounce back, q <- opp(q) (wallBCMap).
+//  - Specular rule (slip_specular = 1; the pass runs when any side is a wall
+//    or a slip wall): wall sides bounce back through wallBCMap, slip sides
+//    reflect through slipWallBCMap (WMRT.cuh:243-264), q <- q mirrored along
+//    the side's axis.  On a node shared by several such sides the assignment
+//    the reference kernel's loop makes last (list position m outer, sides
+//    X-, X+, Y-, Y+, Z-, Z+ inner) stands.
+template <class L>
+__host__ __device__ inline int wall_source(const DevGrid& g, const DevBC& bc, const int c[3], int q) {
+    if (!bc.wall_pass) return -1;
+    int best = -1, src = -1;
+    for (int side = 0; side < 2 * L::D; ++side) {
+        const int a = side >> 1;
+        const int inward = (side & 1) ? -1 : 1;
+        const int t = bc.type[side];
+        if (!(t == BC_WALL || t == BC_SLIP_WALL) || L::e(q, a) != inward ||
+            c[a] != ((side & 1) ? g.dmax[a] : g.dmin[a]))
+            continue;
+        if (!bc.slip_specular) return L::opp(q);
+        const bool slip = t == BC_SLIP_WALL;
+        const int key = (slip ? L::slip_pos(side, q) : L::wall_pos(side, q)) * 6 + side;
+        if (key > best) {
+            best = key;
+            src = slip ? L::mirror(q, a) : L::opp(q);
+        }
+    }
+    return src;
 }
 
 // Index of a ghost-plane (or boundary-plane) cell of side `side` within its
@@ -293,16 +331,9 @@ struct Puller {
     }
 
     __device__ __forceinline__ CT lvl1(const int c[3], int q) const {
-        if (bc.is_wall) {
-#pragma unroll
-            for (int side = 0; side < 2 * L::D; ++side) {
-                const int a = side >> 1;
-                const int inward = (side & 1) ? -1 : 1;
-                const int t = bc.type[side];
-                if ((t == BC_WALL || t == BC_SLIP_WALL) && L::e(q, a) == inward &&
-                    c[a] == ((side & 1) ? g.dmax[a] : g.dmin[a]))
-                    return load(L::opp(q), c);
-            }
+        if (bc.wall_pass) {
+            const int src = wall_source<L>(g, bc, c, q);
+            if (src >= 0) return load(src, c);
         }
         return raw(c, q);
     }

@@ -27,6 +27,12 @@ struct Lattice<9> {
     __host__ __device__ static constexpr int minv_num(int q, int r) { return D2Q9_MINV_NUM[q][r]; }
     __host__ __device__ static constexpr int minv_den(int q) { return D2Q9_MINV_DEN[q]; }
     __host__ __device__ static constexpr int stress(int k) { return D2Q9_STRESS[k]; }
+    // direction q mirrored along axis a
+    __host__ __device__ static constexpr int mirror(int q, int a) { return D2Q9_MIRROR[a][q]; }
+    // position of q in side `side`'s list of the reference's wall map /
+    // slip map (WMRT.cuh:214-264; -1: q does not enter through that side)
+    __host__ __device__ static constexpr int wall_pos(int side, int q) { return D2Q9_WALL_POS[side][q]; }
+    __host__ __device__ static constexpr int slip_pos(int side, int q) { return D2Q9_SLIP_POS[side][q]; }
 };
 
 template <>
@@ -44,6 +50,12 @@ struct Lattice<19> {
     __host__ __device__ static constexpr int minv_num(int q, int r) { return D3Q19_MINV_NUM[q][r]; }
     __host__ __device__ static constexpr int minv_den(int q) { return D3Q19_MINV_DEN[q]; }
     __host__ __device__ static constexpr int stress(int k) { return D3Q19_STRESS[k]; }
+    // direction q mirrored along axis a
+    __host__ __device__ static constexpr int mirror(int q, int a) { return D3Q19_MIRROR[a][q]; }
+    // position of q in side `side`'s list of the reference's wall map /
+    // slip map (WMRT.cuh:214-264; -1: q does not enter through that side)
+    __host__ __device__ static constexpr int wall_pos(int side, int q) { return D3Q19_WALL_POS[side][q]; }
+    __host__ __device__ static constexpr int slip_pos(int side, int q) { return D3Q19_SLIP_POS[side][q]; }
 };
 
 using D2Q9 = Lattice<9>;

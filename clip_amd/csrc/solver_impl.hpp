@@ -1133,8 +1133,9 @@ struct SolverImpl : Impl {
         if constexpr (!std::is_same<ST, CT>::value)
             throw ConfigError("the adjoint runs on F64 or F32 handles");
         for (int s = 0; s < 2 * L::D; ++s)
-            if (cfg.bc_type[s] == CLIP_BC_SLIP_WALL && !bc.is_wall)
-                throw ConfigError("the adjoint does not support a slip side without a wall side");
+            if (cfg.bc_type[s] == CLIP_BC_SLIP_WALL && !bc.wall_pass)
+                throw ConfigError("the adjoint does not support a slip side without a wall side "
+                                  "(unless slip sides reflect specularly)");
     }
     template <class T>
     T* adj_alloc(size_t count) {

@@ -135,6 +135,14 @@ typedef struct clip_config {
        clip_config_derive keeps mobility_override instead, for every case */
     int has_mobility_override;
     double mobility_override;
+    /* slip sides (no reference counterpart in effect): 0 = the reference's
+       behaviour, a slip side takes the wall bounce-back when any side is a
+       wall (kernelHalfBounceBack with wallBCMap, Solver.cu:248-333) and is
+       otherwise left as streamed, since Solver::slipWallBoundary
+       (Solver.cu:1012-1016) is never launched; 1 = slip sides reflect
+       specularly through WMRT::slipWallBCMap (WMRT.cuh:243-264) in the wall
+       pass.  Config key `slipWall = specular`. */
+    int slip_specular;
 } clip_config;
 
 /* Sizes of a created solver. */

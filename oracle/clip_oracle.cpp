@@ -679,8 +679,51 @@ bool on_side(const Oracle& o, int s, int i, int j, int k) {
     return c[ax] == ((s % 2 == 0) ? o.dmin[ax] : o.dmax[ax]);
 }
 
+// The reference's boundary maps as it lists them, sides X-, X+, Y-, Y+, Z-,
+// Z+: WMRT::wallBCMap (WMRT.cuh:214-237) and WMRT::slipWallBCMap
+// (WMRT.cuh:243-264).  kernelHalfBounceBack assigns a_post[b, map[s][m]] =
+// a[b, map[s^1][m]] for list position m outer and side s inner.
+const int kWallMap2[4][3] = {{1, 5, 8}, {3, 7, 6}, {2, 5, 6}, {4, 7, 8}};
+const int kSlipMap2[4][3] = {{1, 5, 8}, {3, 6, 7}, {2, 5, 6}, {4, 8, 7}};
+const int kWallMap3[6][5] = {{1, 9, 7, 13, 11}, {2, 10, 8, 14, 12}, {3, 10, 7, 17, 15},
+                             {4, 9, 8, 18, 16}, {5, 14, 11, 18, 15}, {6, 13, 12, 17, 16}};
+const int kSlipMap3[6][5] = {{1, 9, 7, 13, 11}, {2, 8, 10, 12, 14}, {3, 10, 7, 17, 15},
+                             {4, 8, 9, 16, 18}, {5, 14, 11, 18, 15}, {6, 12, 13, 16, 17}};
+
+int side_map(const Oracle& o, bool slip, int s, int m) {
+    if (o.D == 2) return slip ? kSlipMap2[s][m] : kWallMap2[s][m];
+    return slip ? kSlipMap3[s][m] : kWallMap3[s][m];
+}
+
+// kernelHalfBounceBack with the slip map applied to slip sides and the wall
+// map to wall sides, in the reference kernel's loop order (the opt-in
+// specular slip wall, clip_config::slip_specular).  Launched when any side
+// is a wall or a slip wall.
+void wall_pass_specular(Oracle& o) {
+    if (!(o.isWall || o.isSlip)) return;
+    const int nin = (o.D == 2) ? 3 : 5;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < o.ext[0]; ++i)
+        for (int j = 0; j < o.ext[1]; ++j)
+            for (int k = 0; k < o.ext[2]; ++k) {
+                const size_t id = o.idx(i, j, k);
+                if (!o.interior(i, j, k)) continue;
+                for (int m = 0; m < nin; ++m)
+                    for (int s = 0; s < 2 * o.D; ++s) {
+                        const int t = o.c.bc_type[s];
+                        if (!(t == CLIP_BC_WALL || t == CLIP_BC_SLIP_WALL) || !on_side(o, s, i, j, k)) continue;
+                        const bool slip = MUTANT(10) ? false : t == CLIP_BC_SLIP_WALL;
+                        const int q = side_map(o, slip, s, m);
+                        const int b = MUTANT(11) ? side_map(o, slip, s ^ 1, (m + 1) % nin) : side_map(o, slip, s ^ 1, m);
+                        o.fpost[id * o.Q + q] = o.f[id * o.Q + b];
+                        o.gpost[id * o.Q + q] = o.g[id * o.Q + b];
+                    }
+            }
+}
+
 // kernelHalfBounceBack with wallBCMap (Solver.cu:248-333, launched when isWall).
 void wall_pass(Oracle& o) {
+    if (o.c.slip_specular) return wall_pass_specular(o);
     if (!o.isWall) return;
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < o.ext[0]; ++i)
@@ -1091,6 +1134,17 @@ int orc_lattice(int dim, double* M, double* Minv, int* e, double* w, int* in_lis
         for (int m = 0; m < L.nin; ++m) in_lists[s * L.nin + m] = L.in[s][m];
     *nin = L.nin;
     return L.Q;
+}
+
+// The reference's wall (slip = 0) or slip (slip = 1) map, 2*dim lists of
+// nin directions (the oracle's literal tables).
+int orc_side_map(int dim, int slip, int* out) {
+    const int nin = (dim == 2) ? 3 : 5;
+    for (int s = 0; s < 2 * dim; ++s)
+        for (int m = 0; m < nin; ++m)
+            out[s * nin + m] = (dim == 2) ? (slip ? kSlipMap2[s][m] : kWallMap2[s][m])
+                                          : (slip ? kSlipMap3[s][m] : kWallMap3[s][m]);
+    return nin;
 }
 
 // Single-node kernels exposed for the invariant tests.

@@ -130,3 +130,20 @@ def jet3d_c5(nx=192, ny=192, nz=768, radius=16.0, U=0.05, L=32.0, rho_ratio=100,
             f"inletModes = {modes}\ninletModeAmplitude = [{', '.join(repr(float(a)) for a in amp)}]\n"
             f"inletModePhase = [{', '.join(repr(float(p)) for p in ph)}]\n\n"
             + _geom("sphere", cen, radius) + "\n" + _bc(sides))
+
+
+def box3d(sides, n=(10, 12, 8), radius=3.0, rho_ratio=10, mu_ratio=10):
+    """3D drop mid-box with arbitrary side conditions and gravity -y (N =
+    interior + 2 per axis, ghosts inside N)."""
+    N = tuple(x + 2 for x in n)
+    c = tuple((x - 1) / 2.0 for x in N)
+    return (f"case = drop\nfinalStep = 1000\nN = [{N[0]}, {N[1]}, {N[2]}]\nreferenceLength = {2 * radius}\n"
+            f"We = 5\nRe = 100\ngravity = [0, -1e-5, 0]\ngravityScale = 1e-6\nmobility = 0.02\n"
+            f"interfaceWidth = 4\nmuRatio = {mu_ratio}\nrhoRatio = {rho_ratio}\n\n"
+            + _geom("sphere", c, radius) + "\n" + _bc(sides))
+
+
+def specular(text):
+    """The same configuration with slip sides reflecting specularly
+    (slipWallBCMap) instead of the reference's default."""
+    return "slipWall = specular\n" + text

@@ -58,6 +58,8 @@ def load():
     lib.orc_destroy.restype = None
     lib.orc_destroy.argtypes = [P]
     lib.orc_viscous_force.argtypes = [P, D, ctypes.c_double, D, D]
+    lib.orc_side_map.restype = ctypes.c_int
+    lib.orc_side_map.argtypes = [ctypes.c_int, ctypes.c_int, I]
     lib.orc_init_phi.argtypes = [P, D]
     _lib = lib
     return lib
@@ -150,6 +152,15 @@ def lattice(dim):
     k = nin.value
     return {"M": M.reshape(Q, Q), "Minv": Minv.reshape(Q, Q), "e": e.reshape(Q, 3), "w": w,
             "in": inl[: 2 * dim * k].reshape(2 * dim, k)}
+
+
+def side_map(dim, slip):
+    """The reference's wall map (slip False) or slip map (slip True) as the
+    oracle holds it: one list of in-directions per side."""
+    nin = 3 if dim == 2 else 5
+    out = np.zeros(2 * dim * nin, dtype=np.int32)
+    load().orc_side_map(dim, int(slip), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out.reshape(2 * dim, nin)
 
 
 def gamma(dim, q, u):

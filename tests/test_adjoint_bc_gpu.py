@@ -1,5 +1,5 @@
 """Reverse mode on bounded domains (velocity inlet with azimuthal modes,
-free-convect outflow, Neumann, walls, do-nothing) checked against central
+free-convect outflow, Neumann, walls, specular slip sides, do-nothing) checked against central
 finite differences of the FP64 GPU forward: dL/dsigma, dL/dmobility,
 dL/dA_m of the inlet modes and dL/dphi0 at interface-band nodes next to
 the boundaries, for L = 1/2 |phi_T - phi_target|^2 with the target from a
@@ -19,6 +19,11 @@ CASES = [
     ("rti2d_walls", cases.rti2d(24, 40), 40),
     ("jet3d_c5_20x20x32", cases.jet3d_c5(nx=20, ny=20, nz=32), 30),
     ("jet3d_neumann", cases.jet3d(n=(14, 16, 14), radius=3.0, inlet_side="z-", outlet_type="neumann"), 30),
+    ("box2d_specular_slip", cases.specular(cases.box2d([("x-", "slip wall"), ("x+", "slip wall"),
+                                                        ("y-", "wall"), ("y+", "slip wall")], n=18, ny=20)), 40),
+    ("box2d_specular_slip_only", cases.specular(cases.box2d([("x-", "periodic"), ("x+", "periodic"),
+                                                             ("y-", "slip wall"), ("y+", "slip wall")],
+                                                            n=18, ny=20)), 40),
 ]
 
 

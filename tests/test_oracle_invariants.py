@@ -263,6 +263,84 @@ def test_half_bounce_back_reflects_and_has_zero_wall_flux():
             assert abs(out - inn) < 1e-15
 
 
+def _slip_box(dim):
+    if dim == 2:
+        sides = [("x-", "slip wall"), ("x+", "slip wall"), ("y-", "wall"), ("y+", "slip wall")]
+        return cases.specular(cases.box2d(sides, n=6, ny=5))
+    sides = [("x-", "slip wall"), ("x+", "wall"), ("y-", "slip wall"), ("y+", "slip wall"), ("z-", "periodic"),
+             ("z+", "periodic")]
+    return cases.specular(cases.box3d(sides, n=(5, 6, 4)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_specular_slip_keeps_tangential_momentum_and_has_zero_wall_flux(dim):
+    """Opt-in slip wall: slipWallBCMap (WMRT.cuh:243-264) applied by the
+    half-way wall pass (kernelHalfBounceBack, Solver.cu:248-333).  At a node
+    next to one slip side, the populations entering from the wall carry the
+    mass and the tangential momentum of the same node's populations that left
+    toward it, and the opposite normal momentum (no net flux through the
+    wall).  A wall side of the same box reverses the tangential momentum
+    instead (bounce-back).  Checked on random populations at every node that
+    touches exactly one wall-type side."""
+    cfg = Config.from_string(_slip_box(dim))
+    assert cfg.raw.slip_specular == 1
+    o = Oracle(cfg).init()
+    f, g = _unique_pops(o, 5)
+    o.phase("periodic", "stream", "wall")
+    e = lat(dim)["e"]
+    Q = cfg.q
+    lo = [1, 1, 1 if dim == 3 else 0]
+    hi = [cfg.extents()[a] - 2 if a < dim else 0 for a in range(3)]
+    walls = [s for s in range(2 * dim) if cfg.raw.bc_type[s] in (0, 1)]
+    checked = {0: 0, 1: 0}
+    for name, src, post in (("f", f, o.get("f_post")), ("g", g, o.get("g_post"))):
+        for c in itertools.product(*[range(lo[a], hi[a] + 1) for a in range(3)]):
+            on = [s for s in walls if c[s // 2] == (hi if s % 2 else lo)[s // 2]]
+            if len(on) != 1:
+                continue
+            s = on[0]
+            a, sgn = s // 2, (1 if s % 2 == 0 else -1)
+            slip = cfg.raw.bc_type[s] == 1
+            inn = [q for q in range(Q) if e[q, a] == sgn]
+            out = [q for q in range(Q) if e[q, a] == -sgn]
+            vin = np.array([post[c][q] for q in inn])
+            vout = np.array([src[c][q] for q in out])
+            assert abs(vin.sum() - vout.sum()) < 1e-14, (name, c, "mass")
+            for t in range(dim):
+                pin = float(np.dot(e[inn, t], vin))
+                pout = float(np.dot(e[out, t], vout))
+                if t == a:
+                    assert abs(pin + pout) < 1e-14, (name, c, "normal flux")
+                elif slip:
+                    assert abs(pin - pout) < 1e-14, (name, c, t, "tangential momentum kept")
+                else:
+                    assert abs(pin + pout) < 1e-14, (name, c, t, "tangential momentum reversed")
+            checked[int(slip)] += 1
+    assert checked[0] > 0 and checked[1] > 0, checked
+
+
+def test_reference_slip_side_follows_the_wall_pass_gate():
+    """Without the opt-in, a slip side bounces back exactly like a wall when
+    some side is a wall (kernelHalfBounceBack takes Wall and SlipWall types,
+    wallBoundary runs when isWall, Solver.cu:248-333, 998-1003) and is left as
+    streamed otherwise (slipWallBoundary is never launched)."""
+    walls = [("x-", "periodic"), ("x+", "periodic"), ("y-", "slip wall"), ("y+", "wall")]
+    a = Oracle(Config.from_string(cases.box2d(walls, n=6, ny=5))).init()
+    b = Oracle(Config.from_string(cases.box2d([walls[0], walls[1], ("y-", "wall"), ("y+", "wall")], n=6,
+                                              ny=5))).init()
+    for o in (a, b):
+        _unique_pops(o, 7)
+        o.phase("periodic", "stream", "wall")
+    assert np.array_equal(a.get("f_post"), b.get("f_post"))
+    lone = [("x-", "periodic"), ("x+", "periodic"), ("y-", "slip wall"), ("y+", "do nothing")]
+    c = Oracle(Config.from_string(cases.box2d(lone, n=6, ny=5))).init()
+    _unique_pops(c, 7)
+    c.phase("periodic", "stream")
+    before = c.get("f_post").copy()
+    c.phase("wall")
+    assert np.array_equal(before, c.get("f_post"))
+
+
 def test_neumann_copies_inner_neighbour():
     """kernelNeumann (Solver.cu:462-542)."""
     cfg = Config.from_string(cases.box2d([("x-", "periodic"), ("x+", "periodic"), ("y-", "neumann"), ("y+", "neumann")],
@@ -581,6 +659,15 @@ def test_oracle_and_product_tables_agree(dim):
     assert np.abs(L["Minv"] - np.array([[float(x) for x in r] for r in Mi])).max() < 1e-16
     E = gl.D2Q9_E if dim == 2 else [e[:3] for e in gl.D3Q19_E]
     assert [tuple(r[:dim]) for r in L["e"]] == [tuple(e[:dim]) for e in E]
+    # the kernels' wall / slip side lists (derived from the minus sides) equal
+    # the reference's maps the oracle holds literally
+    wall, slip = gl.side_lists(E, gl.D2Q9_IN_MINUS if dim == 2 else gl.D3Q19_IN_MINUS)
+    assert np.array_equal(np.array(wall), oracle_lib.side_map(dim, False))
+    assert np.array_equal(np.array(slip), oracle_lib.side_map(dim, True))
+    for lists in (wall, slip):  # each side's list holds exactly its in-directions
+        for s, lst in enumerate(lists):
+            a, sgn = s // 2, (1 if s % 2 == 0 else -1)
+            assert sorted(lst) == [q for q in range(len(E)) if E[q][a] == sgn]
 
 
 # ------------------------------------------------------- viscous force

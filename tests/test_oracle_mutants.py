@@ -26,6 +26,10 @@ MUTANTS = {
     7: ("initial g: interface term with a 3M factor", "test_initial_g_moments"),
     8: ("normal: divided by |grad phi|^2", "test_normal_is_unit_gradient_direction"),
     9: ("normal: gradient components permuted", "test_normal_is_unit_gradient_direction"),
+    10: ("specular slip: slip sides bounce back through the wall map",
+         "test_specular_slip_keeps_tangential_momentum_and_has_zero_wall_flux"),
+    11: ("specular slip: each entering direction paired with the next list entry",
+         "test_specular_slip_keeps_tangential_momentum_and_has_zero_wall_flux"),
 }
 
 

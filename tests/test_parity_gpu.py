@@ -128,6 +128,34 @@ def test_box3d_walls():
     assert_fields(gpu, orc)
 
 
+# Opt-in specular slip sides (slipWallBCMap through the half-way wall pass).
+# Nodes shared by slip and wall sides resolve in the reference kernel's loop
+# order, which the oracle runs literally from the reference's map tables and
+# the kernels derive from the minus-side lists; the 3D box with slip and wall
+# sides on every axis exercises every edge and corner combination.
+SLIP_CASES = {
+    "box2d_slip_wall": cases.specular(cases.box2d([("x-", "slip wall"), ("x+", "slip wall"), ("y-", "wall"),
+                                                   ("y+", "slip wall")])),
+    "box2d_slip_outflow": cases.specular(cases.box2d([("x-", "periodic"), ("x+", "periodic"),
+                                                      ("y-", "slip wall"), ("y+", "free convect")])),
+    "box3d_slip_wall_z_periodic": cases.specular(cases.box3d([("x-", "slip wall"), ("x+", "wall"),
+                                                              ("y-", "slip wall"), ("y+", "slip wall"),
+                                                              ("z-", "periodic"), ("z+", "periodic")])),
+    "box3d_slip_every_axis": cases.specular(cases.box3d([("x-", "slip wall"), ("x+", "slip wall"),
+                                                         ("y-", "wall"), ("y+", "slip wall"),
+                                                         ("z-", "slip wall"), ("z+", "wall")], n=(9, 10, 11))),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SLIP_CASES))
+def test_specular_slip_sides(name):
+    cfg, gpu, orc = run_pair(SLIP_CASES[name], 150)
+    assert cfg.raw.slip_specular == 1
+    assert_fields(gpu, orc)
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL, f
+
+
 # Rows of 129..256 interior nodes run the bulk kernels with one block per row
 # (row, plane and their periodic wraps uniform per block); these cases keep
 # that addressing under the same bar as the rest.
@@ -195,8 +223,9 @@ def test_fp32_against_fp64_oracle(name, text, steps, tol):
                                       sides=[("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall"),
                                              ("z-", "periodic"), ("z+", "periodic")]), 60),
     ("jet3d_c5_20x20x32", cases.jet3d_c5(nx=20, ny=20, nz=32), 60),
+    ("box3d_specular_slip", SLIP_CASES["box3d_slip_every_axis"], 60),
 ], ids=["drop2d", "bubble3d", "jet2d", "drop2d_200x40", "drop2d_33", "box2d_25x20", "bubble3d_15",
-        "box3d_walls_15", "jet3d_c5_20x20x32"])
+        "box3d_walls_15", "jet3d_c5_20x20x32", "box3d_specular_slip"])
 def test_mixed_against_fp64_oracle(precision, tol_phi, tol_u, tol_p, name, text, steps):
     cfg, gpu, orc = run_pair(text, steps, precision=precision)
     a_phi, b_phi = gpu.interior("phi"), orc.interior("phi")
