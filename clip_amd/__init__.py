@@ -101,6 +101,15 @@ class Config:
                                                  float(z), ctypes.byref(out)))
         return out.value
 
+    def initial_phi(self):
+        """NSAllen::initialCondition (NsAllen.cu:768-824): the initial phase
+        field on every node, shape (ext_x, ext_y, ext_z, 1)."""
+        ex = self.extents()
+        out = np.empty(ex[0] * ex[1] * ex[2], dtype=np.float64)
+        _abi.check(_abi.load().clip_initial_phi(ctypes.byref(self.raw),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out.reshape(ex[0], ex[1], ex[2], 1)
+
     def interior_shape(self):
         ex = self.extents()
         return tuple(e - 2 for e in ex[: self.raw.dim])
@@ -117,6 +126,10 @@ class Config:
 
 
 InputData = Config
+
+
+def _ptr(addr):
+    return None if addr is None else ctypes.c_void_p(int(addr))
 
 
 class Solver:
@@ -151,6 +164,43 @@ class Solver:
     def init_fresh(self):
         _abi.check(self._lib.clip_init_fresh(self._h))
         return self
+
+    def init_from_phi(self, phi):
+        """NSAllen::deviceInitializer from a phase field on every node
+        (shape (ext_x, ext_y, ext_z[, 1]), CLIP's order, ghosts included)."""
+        ex = self.config.extents()
+        arr = np.ascontiguousarray(phi, dtype=np.float64).ravel()
+        if arr.size != ex[0] * ex[1] * ex[2]:
+            raise ValueError(f"phi must hold {ex[0] * ex[1] * ex[2]} nodes, got {arr.size}")
+        _abi.check(self._lib.clip_init_from_phi(self._h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return self
+
+    # Solver's boundary launchers (Solver.cuh:61-107) on device arrays the
+    # caller owns: FP64, CLIP's AoS layout, ghosts included; arguments are
+    # device addresses (int) or None.
+    def periodic_boundary(self, ndof, dev_a, dev_b=None):
+        _abi.check(self._lib.clip_periodic_boundary(self._h, int(ndof), _ptr(dev_a), _ptr(dev_b)))
+
+    def wall_boundary(self, dev_a, dev_a_post, dev_b=None, dev_b_post=None):
+        _abi.check(self._lib.clip_wall_boundary(self._h, _ptr(dev_a), _ptr(dev_a_post), _ptr(dev_b),
+                                                _ptr(dev_b_post)))
+
+    def slip_wall_boundary(self, dev_a, dev_a_post, dev_b=None, dev_b_post=None):
+        _abi.check(self._lib.clip_slip_wall_boundary(self._h, _ptr(dev_a), _ptr(dev_a_post), _ptr(dev_b),
+                                                     _ptr(dev_b_post)))
+
+    def free_convect_boundary(self, dev_vel, dev_a, dev_a_prev, dev_b=None, dev_b_prev=None):
+        _abi.check(self._lib.clip_free_convect_boundary(self._h, _ptr(dev_vel), _ptr(dev_a), _ptr(dev_a_prev),
+                                                        _ptr(dev_b), _ptr(dev_b_prev)))
+
+    def neumann_boundary(self, dev_a, dev_b=None):
+        _abi.check(self._lib.clip_neumann_boundary(self._h, _ptr(dev_a), _ptr(dev_b)))
+
+    def velocity_boundary(self, dev_c, dev_f, dev_g):
+        _abi.check(self._lib.clip_velocity_boundary(self._h, _ptr(dev_c), _ptr(dev_f), _ptr(dev_g)))
+
+    def mirror_boundary(self, dev_a):
+        _abi.check(self._lib.clip_mirror_boundary(self._h, _ptr(dev_a)))
 
     # nsteps x NSAllen::solve
     def step(self, nsteps=1):

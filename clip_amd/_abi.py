@@ -106,6 +106,8 @@ SIGNATURES = {
     "clip_set_stream": (ctypes.c_int, [_P, _P]),
     "clip_init_fresh": (ctypes.c_int, [_P]),
     "clip_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "clip_init_from_phi": (ctypes.c_int, [_P, _D]),
+    "clip_initial_phi": (ctypes.c_int, [ctypes.POINTER(ClipConfig), _D]),
     "clip_synchronize": (ctypes.c_int, [_P]),
     "clip_get_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
     "clip_set_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
@@ -128,6 +130,13 @@ SIGNATURES = {
     "clip_adjoint_set_nesting": (ctypes.c_int, [_P, ctypes.c_int]),
     "clip_adjoint_inlet_gradient": (ctypes.c_int, [_P, _D, ctypes.c_int]),
     "clip_write_vts": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "clip_periodic_boundary": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
+    "clip_wall_boundary": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "clip_slip_wall_boundary": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "clip_free_convect_boundary": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "clip_neumann_boundary": (ctypes.c_int, [_P, _P, _P]),
+    "clip_velocity_boundary": (ctypes.c_int, [_P, _P, _P, _P]),
+    "clip_mirror_boundary": (ctypes.c_int, [_P, _P]),
     "clip_destroy": (None, [_P]),
     "clip_last_error": (ctypes.c_char_p, []),
 }

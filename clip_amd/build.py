@@ -15,8 +15,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libclip_amd.so")
 DRIVER = os.path.join(HERE, "clip")
 
-# (source, extra flags): the FP64 instantiation is built without FP contraction
-HIP_SOURCES = [("solver.hip", []), ("inst_f64.hip", ["-ffp-contract=off"]), ("inst_f32.hip", []),
+# (source, extra flags): the FP64 instantiation and the C-ABI unit (whose launchers
+# work on FP64 arrays) are built without FP contraction
+HIP_SOURCES = [("solver.hip", ["-ffp-contract=off"]), ("inst_f64.hip", ["-ffp-contract=off"]), ("inst_f32.hip", []),
                ("inst_mixed.hip", []), ("noalloc.hip", [])]
 CPP_SOURCES = ["config.cpp"]
 ARCH = os.environ.get("CLIP_AMD_ARCH", "gfx950")
@@ -62,8 +63,9 @@ def build(verbose=True):
     # C++ driver with CLIP's main-loop shape (config.txt, -resume)
     drv_src = os.path.join(CSRC, "clip_main.cpp")
     if os.path.exists(drv_src):
-        run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC, drv_src,
-             "-L", HERE, "-lclip_amd", "-Wl,-rpath,$ORIGIN", "-o", DRIVER])
+        run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+             "-I", "/opt/rocm/include", drv_src, "-L", HERE, "-lclip_amd", "-L", "/opt/rocm/lib", "-lamdhip64",
+             "-Wl,-rpath,$ORIGIN", "-o", DRIVER])
     return LIB
 
 

@@ -1,6 +1,7 @@
-// clip_main.cpp — the `clip` executable with the reference driver's shape
-// (src/main.cu:39-91): read ./config.txt, fresh start or `-resume`, then the
-// step loop with VTS output, reporting and checkpoints.  Differences: fields
+// clip_main.cpp — the `clip` executable, the reference driver (src/main.cu:39-91)
+// written against include/clip_amd.hpp: read ./config.txt, build the host
+// classes around a DataArray, fresh start or `-resume`, then the step loop
+// with VTS output, reporting and checkpoints.  Differences: fields
 // are copied to the host only on output steps (the reference copies them
 // every step, VTSwriter.cu:72-79); optional `-precision f64|f32|mixed_f16|
 // mixed_bf16`, `-config <path>`, `-steps <n>` and `-novts` flags.
@@ -31,20 +32,26 @@ int main(int argc, char** argv) {
     }
     try {
         clip::InputData inputData(config);
-        clip::Domain domain(inputData);
+        clip::Domain simDomain(inputData);
         clip::Geometry geometry(inputData);
-        clip::Boundary boundary(inputData, domain);
-        clip::NSAllen ns(inputData, domain, boundary, geometry, precision);
+        clip::Boundary boundaryConditions(inputData, simDomain);
+        clip::DataArray dataArray(inputData, simDomain, boundaryConditions);
+        dataArray.createVectors();
+
+        clip::NSAllen nsSolver(inputData, simDomain, dataArray, boundaryConditions, geometry, precision);
         clip::TimeInfo timeInfo(inputData);
-        clip::Reporter reporter(ns, inputData, timeInfo);
-        clip::CheckPointer checkpointer(ns, inputData, timeInfo);
-        clip::VTSwriter vtsWriter(ns, inputData, timeInfo, "results", "results");
+        clip::VTSwriter vtsWriter(dataArray, inputData, simDomain, timeInfo, "results", "results");
+        clip::Reporter reporter(dataArray, inputData, simDomain, timeInfo);
+        clip::CheckPointer checkpointer(dataArray, inputData, simDomain, timeInfo, boundaryConditions);
+        clip::NSAllen& ns = nsSolver;
+
         if (resume) {
             checkpointer.load();
-            ns.macroscopic();  // main.cu:67-71 (already part of the load: skipped)
+            nsSolver.macroscopic();  // already part of the load: skipped
         } else {
-            ns.initialCondition();
-            ns.deviceInitializer();
+            nsSolver.initialCondition();
+            dataArray.updateDevice();
+            nsSolver.deviceInitializer();
         }
         if (vts) vtsWriter.writeToFile();
         const long long final_step = steps_override >= 0 ? timeInfo.getCurrentStep() + steps_override

@@ -31,6 +31,8 @@ struct HipError : std::runtime_error {
 struct Impl {
     virtual ~Impl() {}
     virtual void init_fresh() = 0;
+    // deviceInitializer from a given phi (dense SoA, x fastest, ghosts included)
+    virtual void init_phi(const std::vector<double>& phi) = 0;
     virtual void step(int n) = 0;
     virtual void get_field(int field, std::vector<double>& out) = 0;  // SoA, ghosts included
     virtual void set_field(int field, const std::vector<double>& in) = 0;
@@ -74,6 +76,7 @@ struct Impl {
     virtual double reduce(int field, unsigned* nonfinite) = 0;
     virtual void sync() = 0;
     virtual void set_stream(hipStream_t s) = 0;
+    virtual void* stream_handle() = 0;  // the hipStream_t the handle's work runs on
     virtual void timing(double* total, double* ghost, double* phase, double* fused) const = 0;
     virtual void enable_timing(bool on) = 0;
     virtual void enable_graph(bool on) = 0;

@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "geometry.hpp"
 #include "impl.hpp"
+#include "launchers.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -136,6 +137,79 @@ size_t interior_of(const clip_config& c) {
 
 }  // namespace
 
+// Solver launchers on caller arrays (CLIP AoS, FP64, device memory).
+namespace {
+
+using clipamd::HipError;
+
+bool device_pointer(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// Runs `body(stream, grid, bc)` for a handle's configuration once the arrays
+// are checked to be device memory (optional ones may be null).
+int launcher(clip_solver* s, std::initializer_list<const void*> required, std::initializer_list<const void*> optional,
+             const std::function<void(hipStream_t, const clipamd::AosGrid&, const clipamd::DevBC&)>& body) {
+    if (!s) return set_error(CLIP_ERR_ARG, "null handle");
+    for (const void* p : required)
+        if (!device_pointer(p)) return set_error(CLIP_ERR_ARG, "launcher arrays must be device memory");
+    for (const void* p : optional)
+        if (p && !device_pointer(p)) return set_error(CLIP_ERR_ARG, "launcher arrays must be device memory");
+    return guard([&] {
+        const hipStream_t st = (hipStream_t)s->impl->stream_handle();
+        const clipamd::AosGrid g = clipamd::aos_grid(s->cfg);
+        const clipamd::DevBC bc = clipamd::bc_of_config(s->cfg, s->cfg.dim);
+        body(st, g, bc);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+// A device copy of `n` doubles, freed with the object.
+struct DevCopy {
+    double* p = nullptr;
+    DevCopy(const double* src, long long n, hipStream_t st) {
+        HIP_CHECK(hipMalloc(&p, (size_t)n * sizeof(double)));
+        HIP_CHECK(hipMemcpyAsync(p, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    ~DevCopy() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+template <class Fn>
+void with_lattice(int dim, Fn fn) {
+    if (dim == 2) fn(clipamd::Lattice<9>{});
+    else fn(clipamd::Lattice<19>{});
+}
+
+int half_bb(clip_solver* s, bool slip, const double* dev_a, double* dev_a_post, const double* dev_b,
+                   double* dev_b_post) {
+    if (!dev_b != !dev_b_post) return set_error(CLIP_ERR_ARG, "dev_b and dev_b_post go together");
+    return launcher(s, {dev_a, dev_a_post}, {dev_b, dev_b_post},
+                    [&](hipStream_t st, const clipamd::AosGrid& g, const clipamd::DevBC& bc) {
+                        bool run = false;
+                        for (int k = 0; k < 2 * g.D; ++k)
+                            run |= s->cfg.bc_type[k] == (slip ? CLIP_BC_SLIP_WALL : CLIP_BC_WALL);
+                        if (!run) return;
+                        with_lattice(g.D, [&](auto L) {
+                            using Lt = decltype(L);
+                            clipamd::k_aos_half_bb<<<blocks_for(g.count()), 256, 0, st>>>(
+                                dev_a, dev_a_post, dev_b, dev_b_post, g, bc, clipamd::side_lists<Lt>(slip), Lt::Q);
+                        });
+                    });
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* clip_last_error(void) { return clipamd::last_error_cstr(); }
@@ -163,6 +237,31 @@ int clip_set_stream(clip_solver* s, void* stream) {
 int clip_init_fresh(clip_solver* s) {
     if (!s) return set_error(CLIP_ERR_ARG, "null handle");
     return guard([&] { s->impl->init_fresh(); });
+}
+
+int clip_init_from_phi(clip_solver* s, const double* phi) {
+    if (!s || !phi) return set_error(CLIP_ERR_ARG, "null argument");
+    return guard([&] {
+        unsigned e[3];
+        extents(s->cfg, e);
+        std::vector<double> soa;
+        aos_to_soa(phi, soa, e, 1);
+        s->impl->init_phi(soa);
+    });
+}
+
+int clip_initial_phi(const clip_config* cfg, double* phi) {
+    if (!cfg || !phi) return set_error(CLIP_ERR_ARG, "null argument");
+    if (cfg->dim != 2 && cfg->dim != 3) return set_error(CLIP_ERR_ARG, "dim must be 2 or 3");
+    if (cfg->num_geometries < 0 || cfg->num_geometries > CLIP_MAX_GEOMETRIES)
+        return set_error(CLIP_ERR_ARG, "num_geometries out of range");
+    unsigned e[3];
+    extents(*cfg, e);
+    for (unsigned i = 0; i < e[0]; ++i)
+        for (unsigned j = 0; j < e[1]; ++j)
+            for (unsigned k = 0; k < e[2]; ++k)
+                phi[((size_t)i * e[1] + j) * e[2] + k] = clipamd::initial_phi(*cfg, (int)i, (int)j, (int)k);
+    return CLIP_OK;
 }
 
 int clip_step(clip_solver* s, int nsteps) {
@@ -217,6 +316,126 @@ int clip_macroscopic(clip_solver* s) {
     if (!s) return set_error(CLIP_ERR_ARG, "null handle");
     if (!s->impl->initialized) return set_error(CLIP_ERR_STATE, "solver not initialised");
     return guard([&] { s->impl->macroscopic(); });
+}
+
+
+int clip_periodic_boundary(clip_solver* s, int ndof, double* dev_a, double* dev_b) {
+    if (ndof < 1) return set_error(CLIP_ERR_ARG, "ndof must be positive");
+    return launcher(s, {dev_a}, {dev_b}, [&](hipStream_t st, const clipamd::AosGrid& g, const clipamd::DevBC& bc) {
+        for (int ax = 0; ax < g.D; ++ax) {
+            if (!bc.periodic[ax]) continue;
+            const int t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
+            const long long n = (long long)g.ext[t1] * g.ext[t2] * ndof;
+            for (double* arr : {dev_a, dev_b})
+                if (arr) clipamd::k_aos_periodic<<<blocks_for(n), 256, 0, st>>>(arr, ndof, g, ax);
+        }
+    });
+}
+
+int clip_wall_boundary(clip_solver* s, const double* dev_a, double* dev_a_post, const double* dev_b,
+                       double* dev_b_post) {
+    return half_bb(s, false, dev_a, dev_a_post, dev_b, dev_b_post);
+}
+
+int clip_slip_wall_boundary(clip_solver* s, const double* dev_a, double* dev_a_post, const double* dev_b,
+                            double* dev_b_post) {
+    return half_bb(s, true, dev_a, dev_a_post, dev_b, dev_b_post);
+}
+
+int clip_free_convect_boundary(clip_solver* s, const double* dev_vel, double* dev_a, const double* dev_a_prev,
+                               double* dev_b, const double* dev_b_prev) {
+    if (!dev_b != !dev_b_prev) return set_error(CLIP_ERR_ARG, "dev_b and dev_b_prev go together");
+    return launcher(s, {dev_vel, dev_a, dev_a_prev}, {dev_b, dev_b_prev},
+                    [&](hipStream_t st, const clipamd::AosGrid& g, const clipamd::DevBC& bc) {
+                        if (!bc.is_fc) return;
+                        with_lattice(g.D, [&](auto L) {
+                            using Lt = decltype(L);
+                            const long long n = g.count() * Lt::Q;
+                            DevCopy ca(dev_a, n, st);
+                            std::unique_ptr<DevCopy> cb;
+                            if (dev_b) cb.reset(new DevCopy(dev_b, n, st));
+                            clipamd::k_aos_free_convect<<<blocks_for(g.count()), 256, 0, st>>>(
+                                dev_vel, dev_a, ca.p, dev_a_prev, dev_b, cb ? cb->p : nullptr, dev_b_prev, g, bc,
+                                clipamd::side_lists<Lt>(false), Lt::Q);
+                            HIP_CHECK(hipStreamSynchronize(st));
+                        });
+                    });
+}
+
+int clip_neumann_boundary(clip_solver* s, double* dev_a, double* dev_b) {
+    return launcher(s, {dev_a}, {dev_b}, [&](hipStream_t st, const clipamd::AosGrid& g, const clipamd::DevBC& bc) {
+        if (!bc.is_neumann) return;
+        with_lattice(g.D, [&](auto L) {
+            using Lt = decltype(L);
+            const long long n = g.count() * Lt::Q;
+            DevCopy ca(dev_a, n, st);
+            std::unique_ptr<DevCopy> cb;
+            if (dev_b) cb.reset(new DevCopy(dev_b, n, st));
+            clipamd::k_aos_neumann<<<blocks_for(g.count()), 256, 0, st>>>(dev_a, ca.p, dev_b, cb ? cb->p : nullptr,
+                                                                            g, bc, clipamd::side_lists<Lt>(false),
+                                                                            Lt::Q);
+            HIP_CHECK(hipStreamSynchronize(st));
+        });
+    });
+}
+
+int clip_velocity_boundary(clip_solver* s, double* dev_c, double* dev_f, double* dev_g) {
+    return launcher(s, {dev_c, dev_f, dev_g}, {}, [&](hipStream_t st, const clipamd::AosGrid& g,
+                                                     const clipamd::DevBC& bc) {
+        if (!bc.is_velocity) return;
+        with_lattice(g.D, [&](auto L) {
+            using Lt = decltype(L);
+            for (int side = 0; side < 2 * g.D; ++side) {
+                if (s->cfg.bc_type[side] != CLIP_BC_VELOCITY) continue;
+                // disc flags and inlet velocity of every ghost-plane cell (t1 fastest)
+                const int ax = side / 2, t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
+                const long long cells = (long long)g.ext[t1] * g.ext[t2];
+                std::vector<unsigned char> flag(cells);
+                std::vector<double> U(3 * cells);
+                for (long long t = 0; t < cells; ++t) {
+                    int cc[3];
+                    cc[t1] = (int)(t % g.ext[t1]);
+                    cc[t2] = (int)(t / g.ext[t1]);
+                    cc[ax] = (side & 1) ? g.dmax[ax] + 1 : g.dmin[ax] - 1;
+                    const double z = (g.D == 3) ? (double)cc[2] : 0.0;
+                    flag[t] = clipamd::sdf(s->cfg, 0, cc[0], cc[1], z) <= 0 ? 1 : 0;
+                    clipamd::inlet_velocity(s->cfg, side, cc, &U[3 * t]);
+                }
+                unsigned char* dflag = nullptr;
+                double* dU = nullptr;
+                HIP_CHECK(hipMalloc(&dflag, flag.size()));
+                if (hipMalloc(&dU, U.size() * sizeof(double)) != hipSuccess) {
+                    (void)hipFree(dflag);
+                    throw clipamd::HipError("hipMalloc of the inlet velocity plane failed");
+                }
+                hipError_t e = hipMemcpyAsync(dflag, flag.data(), flag.size(), hipMemcpyHostToDevice, st);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(dU, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, st);
+                if (e == hipSuccess) {
+                    clipamd::k_aos_velocity<Lt><<<blocks_for(cells), 256, 0, st>>>(
+                        dev_c, dev_f, dev_g, g, side, dflag, dU, clipamd::side_lists<Lt>(false));
+                    e = hipStreamSynchronize(st);
+                }
+                (void)hipFree(dflag);
+                (void)hipFree(dU);
+                HIP_CHECK(e);
+            }
+        });
+    });
+}
+
+int clip_mirror_boundary(clip_solver* s, double* dev_a) {
+    return launcher(s, {dev_a}, {}, [&](hipStream_t st, const clipamd::AosGrid& g, const clipamd::DevBC& bc) {
+        if (!bc.mirror_gate) return;
+        for (int side = 0; side < 2 * g.D; ++side) {
+            const int t = s->cfg.bc_type[side];
+            if (!(t == CLIP_BC_WALL || t == CLIP_BC_FREE_CONVECT || t == CLIP_BC_NEUMANN || t == CLIP_BC_SLIP_WALL ||
+                  t == CLIP_BC_DO_NOTHING))
+                continue;
+            const int ax = side / 2, t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
+            clipamd::k_aos_mirror<<<blocks_for((long long)g.ext[t1] * g.ext[t2]), 256, 0, st>>>(dev_a, g, side);
+        }
+    });
 }
 
 int clip_geometry_sdf(const clip_config* cfg, int id, double x, double y, double z, double* out) {

@@ -235,6 +235,7 @@ struct SolverImpl : Impl {
     }
 
     void sync() override { HIP_CHECK(hipStreamSynchronize(stream)); }
+    void* stream_handle() override { return (void*)stream; }
 
     // Arguments of a pass that pulls pop[pull_buf] (encoded relative to
     // phi_ref) into pop[1-pull_buf], u[uin] -> u[1-uin], outflow planes
@@ -418,6 +419,15 @@ struct SolverImpl : Impl {
         for (int k = 0; k < g.ext[2]; ++k)
             for (int j = 0; j < g.ext[1]; ++j)
                 for (int i = 0; i < g.ext[0]; ++i) h[g.idx(i, j, k)] = (CT)initial_phi(cfg, i, j, k);
+        init_with_phi(h);
+    }
+
+    void init_phi(const std::vector<double>& phi) override {
+        if (phi.size() != hn()) throw ConfigError("phi must hold every node, ghosts included");
+        std::vector<CT> h(g.nodes, CT(0));
+        for (int k = 0; k < g.ext[2]; ++k)
+            for (int j = 0; j < g.ext[1]; ++j)
+                for (int i = 0; i < g.ext[0]; ++i) h[g.idx(i, j, k)] = (CT)phi[hidx(i, j, k)];
         init_with_phi(h);
     }
 

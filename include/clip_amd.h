@@ -174,6 +174,51 @@ int clip_config_derive(clip_config* cfg);
  * carrying `id`; 1e10 when none does.  Host only. */
 int clip_geometry_sdf(const clip_config* cfg, int id, double x, double y, double z, double* out);
 
+/* ---- Solver launchers (include/solver/Solver.cuh:61-107) ----------------
+ * The boundary launchers of the reference's Solver class, on the caller's own
+ * arrays: device pointers to FP64 arrays in CLIP's AoS layout with the ghost
+ * ring (((i*ext_y+j)*ext_z+k)*ndof+dof, Domain.cuh:63-68) for the handle's
+ * configuration, which sets the extents, side types, inlet and geometry.
+ * The solver's own step does not use them (it resolves every boundary in its
+ * pull); they serve code written against CLIP's Solver that applies a
+ * boundary to arrays it owns.  Each runs only when the reference's launcher
+ * would (its Boundary flag, Solver.cu:985-1062) and is a no-op otherwise.
+ * dev_b / dev_b_post (or dev_b_prev) may be NULL.  Work is enqueued on the
+ * handle's stream and finished on return.  Host pointers are CLIP_ERR_ARG.
+ * Where the reference's kernel reads what the same launch writes (periodic
+ * edges and corners; inner neighbours of free-convect / Neumann nodes that
+ * are themselves on such a side) these read the values from before the
+ * launch. */
+/* Solver::periodicBoundary<Q> (Solver.cu:75-141, 985-990): ghost layers of
+ * periodic axes take the opposite interior layer, X then Y then Z; ndof =
+ * values per node (Q for populations, 1 for phi). */
+int clip_periodic_boundary(clip_solver* s, int ndof, double* dev_a, double* dev_b);
+/* Solver::wallBoundary<Q, dof> (Solver.cu:248-333, 998-1003): half-way
+ * bounce-back with WMRT::wallBCMap at nodes next to wall and slip-wall sides,
+ * a_post[in] = a[opposite], when some side is a wall. */
+int clip_wall_boundary(clip_solver* s, const double* dev_a, double* dev_a_post, const double* dev_b,
+                       double* dev_b_post);
+/* Solver::slipWallBoundary<Q, dof> (Solver.cu:1012-1016): the same kernel with
+ * WMRT::slipWallBCMap (WMRT.cuh:243-264), when some side is a slip wall. */
+int clip_slip_wall_boundary(clip_solver* s, const double* dev_a, double* dev_a_post, const double* dev_b,
+                            double* dev_b_post);
+/* Solver::freeConvectBoundary<Q, dof> (Solver.cu:342-453, 1025-1029):
+ * a[x, in] = (a_prev[x, in] + |u_n| a[x + n, in]) / (1 + |u_n|) at nodes next
+ * to free-convect sides; dev_vel holds DIM components per node. */
+int clip_free_convect_boundary(clip_solver* s, const double* dev_vel, double* dev_a, const double* dev_a_prev,
+                               double* dev_b, const double* dev_b_prev);
+/* Solver::NeumannBoundary<Q, dof> (Solver.cu:462-542, 1038-1042):
+ * a[x, in] = a[x + n, in] at nodes next to Neumann sides. */
+int clip_neumann_boundary(clip_solver* s, double* dev_a, double* dev_b);
+/* Solver::velocityBoundary (JetBoundary, Solver.cu:550-921, 1058-1062): inlet
+ * populations of f and g on the ghost planes of velocity sides from phi
+ * (dev_c) and the ghost populations, with each side's own inlet velocity. */
+int clip_velocity_boundary(clip_solver* s, double* dev_c, double* dev_f, double* dev_g);
+/* Solver::mirrorBoundary (Solver.cu:927-978, 1048-1052): ghost cells of wall,
+ * free-convect, Neumann, slip and do-nothing sides copy the boundary cell,
+ * when some side is a wall, free-convect, slip wall or Neumann side. */
+int clip_mirror_boundary(clip_solver* s, double* dev_a);
+
 /* ---- solver lifecycle --------------------------------------------------- */
 /* Replaces the object graph built in main.cu:61-65 (Domain, DataArray,
  * NSAllen): allocates every device buffer for the configuration. */
@@ -184,6 +229,15 @@ int clip_set_stream(clip_solver* s, void* hip_stream);
 /* NSAllen::initialCondition + DataArray::updateDevice + NSAllen::deviceInitializer
  * (NsAllen.cu:768-824, DataArray.cu:124-128, NsAllen.cu:897-908). */
 int clip_init_fresh(clip_solver* s);
+/* NSAllen::deviceInitializer (NsAllen.cu:897-908, KernelInitializeDistributions
+ * 137-202) from a phase field the caller gives on every node, ghost ring
+ * included, in CLIP's AoS layout: what the reference computes from dev_c after
+ * initialCondition() and updateDevice() filled it.  clip_init_fresh equals
+ * this call with clip_initial_phi's field. */
+int clip_init_from_phi(clip_solver* s, const double* phi_host);
+/* NSAllen::initialCondition (NsAllen.cu:768-824): the initial phase field of a
+ * configuration on every node, ghosts included (CLIP AoS layout, host). */
+int clip_initial_phi(const clip_config* cfg, double* phi_host);
 /* nsteps x NSAllen::solve (NsAllen.cu:881-892), TimeInfo::increment each. */
 int clip_step(clip_solver* s, int nsteps);
 /* Block until the handle's stream has drained. */

@@ -721,6 +721,29 @@ void wall_pass_specular(Oracle& o) {
             }
 }
 
+// Solver::slipWallBoundary (Solver.cu:1012-1016) as the reference defines it:
+// kernelHalfBounceBack with slipWallBCMap over every Wall and SlipWall side,
+// launched when isSlipWall.  NSAllen::solve never calls it; clip_amd exposes
+// it as a launcher on caller arrays.
+void slip_wall_launcher(Oracle& o) {
+    if (!o.isSlip) return;
+    const int nin = (o.D == 2) ? 3 : 5;
+    for (int i = 0; i < o.ext[0]; ++i)
+        for (int j = 0; j < o.ext[1]; ++j)
+            for (int k = 0; k < o.ext[2]; ++k) {
+                if (!o.interior(i, j, k)) continue;
+                const size_t id = o.idx(i, j, k);
+                for (int m = 0; m < nin; ++m)
+                    for (int s = 0; s < 2 * o.D; ++s) {
+                        const int t = o.c.bc_type[s];
+                        if (!(t == CLIP_BC_WALL || t == CLIP_BC_SLIP_WALL) || !on_side(o, s, i, j, k)) continue;
+                        const int q = side_map(o, true, s, m), b = side_map(o, true, s ^ 1, m);
+                        o.fpost[id * o.Q + q] = o.f[id * o.Q + b];
+                        o.gpost[id * o.Q + q] = o.g[id * o.Q + b];
+                    }
+            }
+}
+
 // kernelHalfBounceBack with wallBCMap (Solver.cu:248-333, launched when isWall).
 void wall_pass(Oracle& o) {
     if (o.c.slip_specular) return wall_pass_specular(o);
@@ -1058,6 +1081,7 @@ int orc_phase(void* h, const char* name) {
     } else if (n == "velocity") velocity_pass(o);
     else if (n == "stream") streaming(o);
     else if (n == "wall") wall_pass(o);
+    else if (n == "slip_wall") slip_wall_launcher(o);
     else if (n == "mirror") mirror_pass(o);
     else if (n == "free_convect") free_convect_pass(o);
     else if (n == "neumann") neumann_pass(o);

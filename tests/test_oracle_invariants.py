@@ -319,6 +319,28 @@ def test_specular_slip_keeps_tangential_momentum_and_has_zero_wall_flux(dim):
     assert checked[0] > 0 and checked[1] > 0, checked
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_slip_wall_launcher_is_the_specular_pass_without_wall_sides(dim):
+    """Solver::slipWallBoundary (Solver.cu:1012-1016) applies slipWallBCMap
+    to every wall-type side; on a box whose wall-type sides are all slip
+    walls that is exactly the pinned specular pass."""
+    if dim == 2:
+        sides = [("x-", "slip wall"), ("x+", "slip wall"), ("y-", "free convect"), ("y+", "slip wall")]
+        text = cases.box2d(sides, n=6, ny=5)
+    else:
+        sides = [("x-", "slip wall"), ("x+", "slip wall"), ("y-", "slip wall"), ("y+", "neumann"),
+                 ("z-", "periodic"), ("z+", "periodic")]
+        text = cases.box3d(sides, n=(5, 6, 4))
+    a = Oracle(Config.from_string(text)).init()
+    b = Oracle(Config.from_string(cases.specular(text))).init()
+    for o in (a, b):
+        _unique_pops(o, 9)
+    a.phase("slip_wall")
+    b.phase("wall")
+    for f in ("f_post", "g_post"):
+        assert np.array_equal(a.interior(f), b.interior(f))
+
+
 def test_reference_slip_side_follows_the_wall_pass_gate():
     """Without the opt-in, a slip side bounces back exactly like a wall when
     some side is a wall (kernelHalfBounceBack takes Wall and SlipWall types,
