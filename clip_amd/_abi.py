@@ -137,6 +137,7 @@ SIGNATURES = {
     "clip_neumann_boundary": (ctypes.c_int, [_P, _P, _P]),
     "clip_velocity_boundary": (ctypes.c_int, [_P, _P, _P, _P]),
     "clip_mirror_boundary": (ctypes.c_int, [_P, _P]),
+    "clip_copy_rate": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, _D]),
     "clip_destroy": (None, [_P]),
     "clip_last_error": (ctypes.c_char_p, []),
 }

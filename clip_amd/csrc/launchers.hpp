@@ -247,4 +247,14 @@ __global__ void k_aos_mirror(double* c, AosGrid g, int side) {
     c[g.node(gc[0], gc[1], gc[2])] = c[g.node(bn[0], bn[1], bn[2])];
 }
 
+// Streaming copy, 16 bytes per lane and one element per thread: the device's
+// practical HBM rate for a pure read + write stream (clip_copy_rate).
+struct alignas(16) Copy16 {
+    unsigned w[4];
+};
+__global__ void k_copy16(const Copy16* __restrict__ a, Copy16* __restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 }  // namespace clipamd

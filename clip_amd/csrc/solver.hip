@@ -438,6 +438,36 @@ int clip_mirror_boundary(clip_solver* s, double* dev_a) {
     });
 }
 
+int clip_copy_rate(unsigned long long bytes, int reps, double* gbs) {
+    if (!gbs || reps < 1 || bytes < 16) return set_error(CLIP_ERR_ARG, "bad argument");
+    return guard([&] {
+        const size_t n = (size_t)(bytes / 16);
+        char* buf = nullptr;
+        HIP_CHECK(hipMalloc(&buf, 2 * n * 16));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipMemset(buf, 1, 2 * n * 16);
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        const auto* src = reinterpret_cast<const clipamd::Copy16*>(buf);
+        auto* dst = reinterpret_cast<clipamd::Copy16*>(buf + n * 16);
+        if (e == hipSuccess) {
+            clipamd::k_copy16<<<blocks, 256>>>(src, dst, n);  // warm-up
+            e = hipEventRecord(e0);
+        }
+        for (int r = 0; r < reps && e == hipSuccess; ++r) clipamd::k_copy16<<<blocks, 256>>>(src, dst, n);
+        if (e == hipSuccess) e = hipEventRecord(e1);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(buf);
+        HIP_CHECK(e);
+        *gbs = 2.0 * (double)n * 16.0 / ((double)ms / reps * 1e-3) / 1e9;
+    });
+}
+
 int clip_geometry_sdf(const clip_config* cfg, int id, double x, double y, double z, double* out) {
     if (!cfg || !out) return set_error(CLIP_ERR_ARG, "null argument");
     if (cfg->num_geometries < 0 || cfg->num_geometries > CLIP_MAX_GEOMETRIES)

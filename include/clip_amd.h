@@ -174,6 +174,12 @@ int clip_config_derive(clip_config* cfg);
  * carrying `id`; 1e10 when none does.  Host only. */
 int clip_geometry_sdf(const clip_config* cfg, int id, double x, double y, double z, double* out);
 
+/* Measurement helper (no counterpart in the reference): GB/s (read + write)
+ * of a 16-byte-per-lane streaming copy of `bytes` on the current device,
+ * timed over `reps` launches: the practical HBM ceiling that bench.py
+ * compares the step kernels with.  Allocates and frees 2 x bytes. */
+int clip_copy_rate(unsigned long long bytes, int reps, double* gbs);
+
 /* ---- Solver launchers (include/solver/Solver.cuh:61-107) ----------------
  * The boundary launchers of the reference's Solver class, on the caller's own
  * arrays: device pointers to FP64 arrays in CLIP's AoS layout with the ghost
