@@ -14,14 +14,21 @@
 //   P^{n+1}          f_post / g_post on the outflow boundary nodes
 // The reverse sweep of one step runs:
 //   k_adj_node   node VJP (node.hpp) at every interior node (+ the adjoint
-//                of P^{n+1} on outflow boundary nodes)
+//                of P^{n+1} on outflow boundary nodes); a bulk node scatters
+//                its f_post adjoint through the transpose of its pull (a
+//                permutation) straight into the adjoint of S^n's f*
 //   k_adj_phi    transpose of the stencil and of the phase sum's consumer:
-//                adjoint of phi' at every cell (interior and ring)
+//                adjoint of phi' at every cell (interior and ring); a bulk
+//                node then scatters its g_post adjoint plus its phi' adjoint
+//                into the adjoint of S^n's g*
 //   k_adj_ring   transpose of the ring copy: adjoint of phi^n (host-built map)
-//   k_adj_pull   transpose of the bulk pull (gather, a permutation)
 //   k_adj_csr_*  transposes of the boundary-layer chains (host-built lists):
 //                populations, outflow planes, inlet values, and u through the
 //                free-convect weights
+// Every entry of the adjoint of S^n's populations has exactly one reader in
+// the step: a bulk node (its scatter writes the entry), a boundary-layer node
+// (it zeroes the entry, the row lists then add the chain's terms) or none
+// (the entry's own boundary-layer node zeroes it).
 #pragma once
 
 #include "adjoint_maps.hpp"
@@ -71,8 +78,10 @@ struct AdjArgs {
     const CT* G;     // g* of S^n
     const CT* phin;  // phi^{n+1} with the ghost ring of phi^n (or phi0 at init)
     const CT* uo;    // u^n
-    CT* lF;          // in: adjoint of f*^{n+1}; out: adjoint of f*^n
-    CT* lG;          // in: adjoint of g*^{n+1}; out: adjoint of g*^n
+    const CT* lF;    // adjoint of f*^{n+1}
+    const CT* lG;    // adjoint of g*^{n+1}
+    CT* lFo;         // adjoint of f*^n (written through the pull's transpose)
+    CT* lGo;         // adjoint of g*^n
     CT* lU;          // in: adjoint of u^{n+1};  out: adjoint of u^n
     CT* lPhi;        // in: adjoint of phi^{n+1}; out: adjoint of phi^n (interior)
     CT* lfp;         // scratch Q x nodes: adjoint of f_post
@@ -144,6 +153,50 @@ __device__ __forceinline__ void block_sum4(double v0, double v1, double v2, doub
 
 constexpr int ADJ_TB = 256;
 
+// Pull source of population q at node c: c - e_q, wrapped along periodic
+// axes (every source of a bulk node is interior).
+template <class L>
+__device__ __forceinline__ long long pull_source(const DevGrid& g, const int c[3], int q) {
+    const int sx = wrap_axis(g, 0, c[0] - L::ex(q)), sy = wrap_axis(g, 1, c[1] - L::ey(q));
+    const int sz = (L::D == 3) ? wrap_axis(g, 2, c[2] - L::ez(q)) : c[2];
+    return g.idx(sx, sy, sz);
+}
+
+// Boundary-layer node c, direction q: zero the adjoint entries of S^n whose
+// only reader it is, or which have none.  The reader of entry (y, q) is
+// y + e_q wrapped along axes periodic on both ends: for y = c - e_q that is c
+// itself; entry (c, q) has no reader when c + e_q leaves the domain.
+template <class L, class CT>
+__device__ __forceinline__ void zero_owned_entries(const AdjArgs<CT>& a, const int c[3], long long n, int q) {
+    const DevGrid& g = a.g;
+    int y[3] = {c[0] - L::ex(q), c[1] - L::ey(q), c[2] - L::ez(q)};
+    int x[3] = {c[0] + L::ex(q), c[1] + L::ey(q), c[2] + L::ez(q)};
+#pragma unroll
+    for (int ax = 0; ax < L::D; ++ax)
+        if (a.bc.both_periodic[ax]) {
+            y[ax] = wrap_axis(g, ax, y[ax]);
+            x[ax] = wrap_axis(g, ax, x[ax]);
+        }
+    if (g.interior(y, L::D)) {
+        const long long t = (long long)q * g.nodes + g.idx(y[0], y[1], y[2]);
+        a.lFo[t] = CT(0);
+        a.lGo[t] = CT(0);
+    }
+    if (!g.interior(x, L::D)) {
+        a.lFo[(long long)q * g.nodes + n] = CT(0);
+        a.lGo[(long long)q * g.nodes + n] = CT(0);
+    }
+}
+
+// Transpose of the bulk pull for g at bulk node c: (source of q, q) <- the
+// g_post adjoint plus the phase sum's share (phi' = sum_q g_post).
+template <class L, class CT>
+__device__ __forceinline__ void scatter_g(const AdjArgs<CT>& a, const int c[3], long long n, CT lphi) {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q)
+        a.lGo[(long long)q * a.g.nodes + pull_source<L>(a.g, c, q)] = a.lgp[(long long)q * a.g.nodes + n] + lphi;
+}
+
 template <class L, class CT>
 __global__ __launch_bounds__(ADJ_TB) void k_adj_node(AdjArgs<CT> a) {
     constexpr int Q = L::Q, D = L::D;
@@ -196,10 +249,22 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_node(AdjArgs<CT> a) {
                     }
             }
         }
+        if (!shell) {
+            // transpose of the bulk pull: (source of q, q) <- this node's f_post adjoint
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            a.lfp[(long long)q * g.nodes + n] = o.lfp[q];
-            a.lgp[(long long)q * g.nodes + n] = o.lgp[q];
+            for (int q = 0; q < Q; ++q) {
+                a.lFo[(long long)q * g.nodes + pull_source<L>(g, c, q)] = o.lfp[q];
+                a.lgp[(long long)q * g.nodes + n] = o.lgp[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                a.lfp[(long long)q * g.nodes + n] = o.lfp[q];
+                a.lgp[(long long)q * g.nodes + n] = o.lgp[q];
+                // the entry this node reads (q from its source), and the node's
+                // own entry q when nothing reads it (q leaves the domain)
+                zero_owned_entries<L>(a, c, n, q);
+            }
         }
         a.lst[n] = o.lc0;
 #pragma unroll
@@ -255,15 +320,40 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_shell_post(AdjArgs<CT> a, ShellD
 #undef CLIP_ADJ_DIR
 }
 
+// acc[k] += sum over blocks of part[4 b + k], run by a whole block
+// (blockDim.x a power of two, 8..256; every thread must call it).  Thread t
+// sums k = t % 4 over b = t / 4, t / 4 + blockDim.x / 4, ... and the per-thread
+// sums meet in a fixed tree, so the result does not depend on timing.
+__device__ __forceinline__ void accumulate4(const double* part, int blocks, double* acc) {
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int b = t >> 2; b < blocks; b += (int)(blockDim.x >> 2)) s += part[4 * b + (t & 3)];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = (int)blockDim.x >> 1; w >= 4; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];  // w is a multiple of 4: t and t + w share k
+        __syncthreads();
+    }
+    if (t < 4) acc[t] += sh[t];
+}
+
 // Transpose of the stencil at every cell y of phi' (interior and ghost ring):
 // lall[y] = lPhi[y] (adjoint from later steps) + the node's own phi term
 // (interior y) + the stencil weights of every interior node z within one
-// cell of y.
-template <class L, class CT>
-__global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a, CT* lall) {
+// cell of y.  With SCATTER (a reverse step; not at the initial state) bulk
+// cells then scatter their g adjoint (scatter_g), and the launch's one extra
+// block, its first, folds k_adj_node's `fold` partials into acc while the
+// others run.
+template <class L, class CT, bool SCATTER>
+__global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a, CT* lall, double* acc, int fold) {
     constexpr int D = L::D;
     const DevGrid& g = a.g;
-    const long long y = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (SCATTER && blockIdx.x == 0) {
+        accumulate4(a.part, fold, acc);
+        return;
+    }
+    const long long y = (long long)(blockIdx.x - (SCATTER ? 1 : 0)) * blockDim.x + threadIdx.x;
     int c[3];
     if (y >= g.nodes || !g.coords(y, c)) return;
     CT total = a.lPhi[y];
@@ -283,19 +373,27 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a, CT* lall) {
                 total += v;
             }
     lall[y] = total;
+    if (SCATTER && g.interior(c, D) && !in_shell<D>(g, a.bc, c)) scatter_g<L>(a, c, y, total);
 }
 
 // Fully periodic domains: the ring of phi' is the periodic wrap of phi^n's
 // interior, so the stencil transpose gathers with wrapped reads in one
 // kernel at interior nodes: lphin = the adjoint of phi' and, with LAG, the
 // reads that hit the lagged ring go to lPhi (the adjoint of phi^n); without
-// LAG (the initial state, whose ring wraps phi0) they stay in lphin.
+// LAG (the initial state, whose ring wraps phi0) they stay in lphin.  A
+// reverse step (LAG) then scatters each node's g adjoint (scatter_g), and the
+// launch's one extra block, its first, folds k_adj_node's `fold` partials
+// into acc while the others run.
 template <class L, class CT, bool LAG>
-__global__ __launch_bounds__(ADJ_TB) void k_adj_phi_periodic(AdjArgs<CT> a) {
+__global__ __launch_bounds__(ADJ_TB) void k_adj_phi_periodic(AdjArgs<CT> a, double* acc, int fold) {
     constexpr int D = L::D;
     const DevGrid& g = a.g;
+    if (LAG && blockIdx.x == 0) {
+        accumulate4(a.part, fold, acc);
+        return;
+    }
     int c[3];
-    if (!interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+    if (!interior_node<D>(g, (long long)(blockIdx.x - (LAG ? 1 : 0)) * blockDim.x + threadIdx.x, c)) return;
     const long long n = g.idx(c[0], c[1], c[2]);
     CT total = a.lPhi[n] + a.lst[n];
     CT lag = CT(0);
@@ -317,7 +415,10 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi_periodic(AdjArgs<CT> a) {
                 else lag += v;
             }
     a.lphin[n] = total;
-    if (LAG) a.lPhi[n] = lag;
+    if (LAG) {
+        a.lPhi[n] = lag;
+        scatter_g<L>(a, c, n, total);
+    }
 }
 
 // Transpose of a copy map (host-built rows: target = a cell of the source
@@ -338,57 +439,6 @@ __global__ void k_adj_fill(CT* x, long long n, CT v) {
     if (i < n) x[i] = v;
 }
 
-// acc[k] += sum over blocks of part[4 b + k], run by a whole block
-// (blockDim.x a power of two, 8..256; every thread must call it).  Thread t
-// sums k = t % 4 over b = t / 4, t / 4 + blockDim.x / 4, ... and the per-thread
-// sums meet in a fixed tree, so the result does not depend on timing.
-__device__ __forceinline__ void accumulate4(const double* part, int blocks, double* acc) {
-    __shared__ double sh[256];
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int b = t >> 2; b < blocks; b += (int)(blockDim.x >> 2)) s += part[4 * b + (t & 3)];
-    sh[t] = s;
-    __syncthreads();
-    for (int w = (int)blockDim.x >> 1; w >= 4; w >>= 1) {
-        if (t < w) sh[t] += sh[t + w];  // w is a multiple of 4: t and t + w share k
-        __syncthreads();
-    }
-    if (t < 4) acc[t] += sh[t];
-}
-
-// Transpose of the pull: adjoint of f*, g* of S^n at interior node y.
-
-template <class L, class CT>
-__global__ __launch_bounds__(ADJ_TB) void k_adj_pull(AdjArgs<CT> a, double* acc, int blocks) {
-    constexpr int Q = L::Q, D = L::D;
-    const DevGrid& g = a.g;
-    // the launch has one block more than the pull needs: that block folds
-    // k_adj_node's partials, next to the pull rather than after it
-    if (blockIdx.x == (unsigned)blocks) {
-        accumulate4(a.part, blocks, acc);
-        return;
-    }
-    int c[3];
-    if (!interior_node<D>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-    const long long n = g.idx(c[0], c[1], c[2]);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        // the bulk node that pulls (n, q): n + e_q, wrapped along axes periodic
-        // on both ends; boundary-layer readers come from the row lists
-        int x3[3] = {c[0] + L::ex(q), c[1] + L::ey(q), c[2] + L::ez(q)};
-#pragma unroll
-        for (int ax = 0; ax < D; ++ax)
-            if (a.bc.both_periodic[ax]) x3[ax] = wrap_axis(g, ax, x3[ax]);
-        CT vf = CT(0), vg = CT(0);
-        if (g.interior(x3, D) && !in_shell<D>(g, a.bc, x3)) {
-            const long long x = g.idx(x3[0], x3[1], x3[2]);
-            vf = a.lfp[(long long)q * g.nodes + x];
-            vg = a.lgp[(long long)q * g.nodes + x] + a.lphin[x];
-        }
-        a.lF[(long long)q * g.nodes + n] = vf;
-        a.lG[(long long)q * g.nodes + n] = vg;
-    }
-}
 
 // Weight of a boundary-chain term (1, or a free-convect weight from u^n).
 template <class CT>
@@ -419,8 +469,8 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_csr_pop(AdjArgs<CT> a, AdjRows r
     if (row >= r.rows) return;
     CT sf, sg;
     row_sums(a, r, row, sf, sg);
-    a.lF[r.target[row]] += sf;
-    a.lG[r.target[row]] += sg;
+    a.lFo[r.target[row]] += sf;
+    a.lGo[r.target[row]] += sg;
 }
 
 // Readers of outflow-plane entries: the adjoint of P^n (every entry has a

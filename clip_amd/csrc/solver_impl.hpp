@@ -1063,7 +1063,8 @@ struct SolverImpl : Impl {
     size_t adj_capacity = 0;  // states in the pool
     size_t adj_bytes = 0;
     CT* adj_pool = nullptr;
-    CT *lF = nullptr, *lG = nullptr, *lU = nullptr, *lPhi = nullptr, *lfp = nullptr, *lgp = nullptr,
+    CT *lF = nullptr, *lG = nullptr, *lF2 = nullptr, *lG2 = nullptr, *lU = nullptr, *lPhi = nullptr, *lfp = nullptr,
+       *lgp = nullptr,
        *lst = nullptr, *lphin = nullptr;
     CT* lP[2] = {nullptr, nullptr};  // adjoint of the outflow planes (f sides, then g sides), ping-pong
     int lpi = 0;                     // lP[lpi] holds the adjoint of the planes of the state after the step
@@ -1346,6 +1347,8 @@ struct SolverImpl : Impl {
             const size_t nq = (size_t)g.nodes * QL;
             lF = adj_alloc_grid<CT>(nq);
             lG = adj_alloc_grid<CT>(nq);
+            lF2 = adj_alloc_grid<CT>(nq);
+            lG2 = adj_alloc_grid<CT>(nq);
             lfp = adj_alloc_grid<CT>(nq);
             lgp = adj_alloc_grid<CT>(nq);
             lU = adj_alloc_grid<CT>((size_t)g.nodes * L::D);
@@ -1399,6 +1402,8 @@ struct SolverImpl : Impl {
         A.uo = st_u(st);
         A.lF = lF;
         A.lG = lG;
+        A.lFo = lF2;
+        A.lGo = lG2;
         A.lU = lU;
         A.lPhi = lPhi;
         A.lfp = lfp;
@@ -1429,9 +1434,11 @@ struct SolverImpl : Impl {
         if (r.rows) fn((unsigned)((r.rows + ADJ_TB - 1) / ADJ_TB));
     }
 
-    // Reverse of the step from kept state `st` into `after`.  The partial sums
-    // of the parameter adjoints are folded into the accumulator by
-    // k_adj_pull's extra last block.
+    // Reverse of the step from kept state `st` into `after`: lF / lG (the
+    // adjoint of the later state's populations) -> lF2 / lG2 (that of st's),
+    // then the two swap.  The partial sums of k_adj_node's parameter
+    // adjoints are folded into the accumulator by an extra block of the
+    // k_adj_phi launch.
     void adjoint_step(CT* st, CT* after) {
         const AdjArgs<CT> A = adj_args(st, after);
         const unsigned blocks = (unsigned)((interior_count() + ADJ_TB - 1) / ADJ_TB);
@@ -1443,17 +1450,15 @@ struct SolverImpl : Impl {
         k_adj_node<L, CT><<<blocks, ADJ_TB, 0, stream>>>(A);
         launch_check("k_adj_node");
         if (!shell_nodes()) {
-            k_adj_phi_periodic<L, CT, true><<<blocks, ADJ_TB, 0, stream>>>(A);
+            k_adj_phi_periodic<L, CT, true><<<blocks + 1, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
             launch_check("k_adj_phi");
         } else {
-            k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
+            k_adj_phi<L, CT, true><<<nb + 1, ADJ_TB, 0, stream>>>(A, lphin, adj_acc, (int)blocks);
             launch_check("k_adj_phi");
             fill(lPhi, g.nodes);
             launch_rows(rows_ring,
                         [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_ring, lphin, lPhi); });
         }
-        k_adj_pull<L, CT><<<blocks + 1, ADJ_TB, 0, stream>>>(A, adj_acc, (int)blocks);
-        launch_check("k_adj_pull");
         launch_rows(rows_pop, [&](unsigned b) { k_adj_csr_pop<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_pop); });
         if (prev_elems) {
             fill(lP[1 - lpi], 2 * prev_elems);
@@ -1464,6 +1469,8 @@ struct SolverImpl : Impl {
         launch_rows(rows_u, [&](unsigned b) { k_adj_csr_u<CT><<<b, ADJ_TB, 0, stream>>>(A, rows_u); });
         launch_check("k_adj_csr");
         lpi = 1 - lpi;
+        std::swap(lF, lF2);
+        std::swap(lG, lG2);
     }
 
     // Steps s0 .. s1-1 in reverse, from kept state `sa` (state s0); `after`
@@ -1512,6 +1519,8 @@ struct SolverImpl : Impl {
         const size_t nq = (size_t)g.nodes * QL;
         fill(lF, (long long)nq);
         fill(lG, (long long)nq);
+        fill(lF2, (long long)nq);
+        fill(lG2, (long long)nq);
         fill(lU, g.nodes * L::D);
         fill(lst, g.nodes * (L::D + 2));
         fill(lphin, g.nodes);
@@ -1543,12 +1552,12 @@ struct SolverImpl : Impl {
         launch_check("k_adj_accumulate");
         std::vector<double> la, lr;
         if (!shell_nodes()) {
-            k_adj_phi_periodic<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A);
+            k_adj_phi_periodic<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A, nullptr, 0);
             launch_check("k_adj_phi");
             download(lphin, (size_t)g.nodes, la);
             lr.assign((size_t)g.nodes, 0.0);
         } else {
-            k_adj_phi<L, CT><<<nb, ADJ_TB, 0, stream>>>(A, lphin);
+            k_adj_phi<L, CT, false><<<nb, ADJ_TB, 0, stream>>>(A, lphin, nullptr, 0);
             launch_check("k_adj_phi");
             fill(lPhi, g.nodes);
             launch_rows(rows_init,
