@@ -303,13 +303,13 @@ class Solver:
         checkpoint_every steps, and with 0 < inner_every < checkpoint_every a
         second level inside each recomputed segment)."""
         _abi.check(self._lib.clip_adjoint_set_nesting(self._h, int(inner_every)))
-        shape = self.config.interior_shape()
-        a = np.ascontiguousarray(np.asarray(phi0, dtype=np.float64).reshape(shape).transpose()).ravel()
+        a = self._interior_array(phi0)
         out = np.empty_like(a)
         _abi.check(self._lib.clip_adjoint_forward(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                   int(nsteps), int(checkpoint_every),
-                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-        return out.reshape(shape[::-1]).transpose().copy()
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                  _abi.INTERIOR_X_SLOWEST))
+        return out
 
     def adjoint_backward(self, grad_phi, normal_floor=0.0):
         """dL/dphi0, dL/dsigma, dL/dmobility for dL/dphi(last step) = grad_phi.
@@ -317,14 +317,19 @@ class Solver:
         bounded derivative of the interface normal on flat plateaus
         (clip_adjoint_set_normal_floor)."""
         _abi.check(self._lib.clip_adjoint_set_normal_floor(self._h, float(normal_floor)))
-        shape = self.config.interior_shape()
-        g = np.ascontiguousarray(np.asarray(grad_phi, dtype=np.float64).reshape(shape).transpose()).ravel()
+        g = self._interior_array(grad_phi)
         out = np.empty_like(g)
         gs, gm = ctypes.c_double(), ctypes.c_double()
         _abi.check(self._lib.clip_adjoint_backward(self._h, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                                   ctypes.byref(gs), ctypes.byref(gm)))
-        return out.reshape(shape[::-1]).transpose().copy(), gs.value, gm.value
+                                                   ctypes.byref(gs), ctypes.byref(gm), _abi.INTERIOR_X_SLOWEST))
+        return out, gs.value, gm.value
+
+    def _interior_array(self, x):
+        """x as a C-ordered float64 array of the interior shape (index [i, j(,
+        k)]: the library's x-slowest interior order, no copy when x already
+        is one)."""
+        return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(self.config.interior_shape()))
 
     def adjoint_inlet_gradient(self):
         """dL/dA_m of the inlet modes (inletModeAmplitude) from the last

@@ -21,6 +21,8 @@ PRECISION = {"f64": 0, "f32": 1, "mixed_f16": 2, "mixed_bf16": 3}
 FIELD = {"phi": 0, "rho": 1, "p": 2, "u": 3, "mu": 4, "dphi": 5, "normal": 6,
          "f": 7, "g": 8, "f_post": 9, "g_post": 10}
 LAYOUT = {"aos": 0, "soa": 1}
+INTERIOR_X_FASTEST = 0  # interior arrays of the reverse-mode calls: i + nx (j + ny k)
+INTERIOR_X_SLOWEST = 1  # (i ny + j) nz + k, numpy's C order for index [i, j, k]
 
 
 class ClipConfig(ctypes.Structure):
@@ -124,8 +126,8 @@ SIGNATURES = {
     "clip_last_step_timing": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "clip_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "clip_enable_graph": (ctypes.c_int, [_P, ctypes.c_int]),
-    "clip_adjoint_forward": (ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int, _D]),
-    "clip_adjoint_backward": (ctypes.c_int, [_P, _D, _D, _D, _D]),
+    "clip_adjoint_forward": (ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int]),
+    "clip_adjoint_backward": (ctypes.c_int, [_P, _D, _D, _D, _D, ctypes.c_int]),
     "clip_adjoint_set_normal_floor": (ctypes.c_int, [_P, ctypes.c_double]),
     "clip_adjoint_set_nesting": (ctypes.c_int, [_P, ctypes.c_int]),
     "clip_adjoint_inlet_gradient": (ctypes.c_int, [_P, _D, ctypes.c_int]),

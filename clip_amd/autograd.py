@@ -2,11 +2,13 @@
 
     phi_T = CoupledSteps.apply(phi0, sigma, mobility, config, nsteps, checkpoint_every, precision)
 
-phi0 (interior shape, index [i, j(, k)]), sigma and mobility are tensors;
-the forward runs nsteps coupled steps on the GPU from the case's initial
-state built on phi0 (clip_adjoint_forward) and the backward runs the
-reverse sweep (clip_adjoint_backward): gradients with respect to phi0,
-sigma and mobility.  Fully periodic domains, "f64" or "f32".  normal_floor
+phi0 (interior shape, index [i, j(, k)], passed to the library in that
+C order), sigma and mobility are tensors; the forward runs nsteps coupled
+steps on the GPU from the case's initial state built on phi0
+(clip_adjoint_forward) and the backward runs the reverse sweep
+(clip_adjoint_backward): gradients with respect to phi0, sigma and
+mobility.  Periodic, wall, velocity-inlet, free-convect, Neumann and
+do-nothing sides; "f64" or "f32".  normal_floor
 (default 0.1) bounds the derivative of the interface normal on flat
 plateaus (clip_adjoint_set_normal_floor); 0 gives the exact derivative.
 """

@@ -439,6 +439,40 @@ __global__ void k_adj_fill(CT* x, long long n, CT v) {
     if (i < n) x[i] = v;
 }
 
+// Node of element t of an FP64 interior array: x fastest (t = i + nx (j +
+// ny k)) or x slowest (t = (i ny + j) nz + k, a C array [nx][ny][nz]); the
+// interior extents are dmax - dmin + 1 per axis (1 on z in 2D).
+__device__ __forceinline__ long long interior_node(const DevGrid& g, long long t, int xslow) {
+    const long long n0 = g.dmax[0] - g.dmin[0] + 1, n1 = g.dmax[1] - g.dmin[1] + 1, n2 = g.dmax[2] - g.dmin[2] + 1;
+    long long i, j, k;
+    if (xslow) {
+        k = t % n2;
+        j = (t / n2) % n1;
+        i = t / (n2 * n1);
+    } else {
+        i = t % n0;
+        j = (t / n0) % n1;
+        k = t / (n0 * n1);
+    }
+    return g.idx((int)i + g.dmin[0], (int)j + g.dmin[1], (int)k + g.dmin[2]);
+}
+
+// Interior array -> node array (the other nodes are left as they are).
+template <class CT>
+__global__ void k_adj_from_interior(CT* dst, const double* src, DevGrid g, long long count, int xslow) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) dst[interior_node(g, t, xslow)] = (CT)src[t];
+}
+
+// Node arrays -> interior array: a (+ b where b is given) at each interior node.
+template <class CT>
+__global__ void k_adj_to_interior(double* dst, const CT* a, const CT* b, DevGrid g, long long count, int xslow) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const long long n = interior_node(g, t, xslow);
+    dst[t] = b ? (double)a[n] + (double)b[n] : (double)a[n];
+}
+
 
 // Weight of a boundary-chain term (1, or a free-convect weight from u^n).
 template <class CT>

@@ -80,16 +80,16 @@ struct Impl {
     virtual void timing(double* total, double* ghost, double* phase, double* fused) const = 0;
     virtual void enable_timing(bool on) = 0;
     virtual void enable_graph(bool on) = 0;
-    // Reverse mode from an interior phi0 (x fastest) over nsteps steps.
-    virtual void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every,
-                                 std::vector<double>& phi_out) = 0;
+    // Reverse mode from an interior phi0 over nsteps steps; interior arrays
+    // are host FP64, x fastest (xslow = 0) or x slowest (xslow = 1).
+    virtual void adjoint_forward(const double* phi0, int xslow, int nsteps, int every, double* phi_out) = 0;
     virtual void adjoint_options(double normal_floor) = 0;
     // Inner checkpoint interval of the reverse sweep (0: one level).
     virtual void adjoint_nesting(int inner) = 0;
     // dL/dA_m of the inlet modes after the last adjoint_backward.
     virtual void adjoint_inlet_grad(std::vector<double>& out) = 0;
-    virtual void adjoint_backward(const std::vector<double>& grad_out, std::vector<double>& grad_phi0,
-                                  double* grad_sigma, double* grad_mobility) = 0;
+    virtual void adjoint_backward(const double* grad_out, int xslow, double* grad_phi0, double* grad_sigma,
+                                  double* grad_mobility) = 0;
     int D = 2, Q = 9;
     unsigned long long bytes = 0;
     long long step_count = 0;

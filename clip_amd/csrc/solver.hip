@@ -531,26 +531,21 @@ int clip_enable_timing(clip_solver* s, int on) {
     return CLIP_OK;
 }
 
-int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out) {
+int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out,
+                         int layout) {
     if (!s || !phi0 || !phi_out) return set_error(CLIP_ERR_ARG, "null argument");
     if (nsteps < 1 || checkpoint_every < 1) return set_error(CLIP_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
-    return guard([&] {
-        const size_t n = interior_of(s->cfg);
-        std::vector<double> in(phi0, phi0 + n), out;
-        s->impl->adjoint_forward(in, nsteps, checkpoint_every, out);
-        std::memcpy(phi_out, out.data(), n * sizeof(double));
-    });
+    if (layout != CLIP_INTERIOR_X_FASTEST && layout != CLIP_INTERIOR_X_SLOWEST)
+        return set_error(CLIP_ERR_ARG, "unknown interior layout");
+    return guard([&] { s->impl->adjoint_forward(phi0, layout, nsteps, checkpoint_every, phi_out); });
 }
 
 int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* grad_phi0, double* grad_sigma,
-                          double* grad_mobility) {
+                          double* grad_mobility, int layout) {
     if (!s || !grad_phi_out || !grad_phi0) return set_error(CLIP_ERR_ARG, "null argument");
-    return guard([&] {
-        const size_t n = interior_of(s->cfg);
-        std::vector<double> in(grad_phi_out, grad_phi_out + n), out;
-        s->impl->adjoint_backward(in, out, grad_sigma, grad_mobility);
-        std::memcpy(grad_phi0, out.data(), n * sizeof(double));
-    });
+    if (layout != CLIP_INTERIOR_X_FASTEST && layout != CLIP_INTERIOR_X_SLOWEST)
+        return set_error(CLIP_ERR_ARG, "unknown interior layout");
+    return guard([&] { s->impl->adjoint_backward(grad_phi_out, layout, grad_phi0, grad_sigma, grad_mobility); });
 }
 
 int clip_adjoint_set_nesting(clip_solver* s, int inner_every) {

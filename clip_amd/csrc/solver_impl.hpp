@@ -436,6 +436,10 @@ struct SolverImpl : Impl {
     void init_with_phi(const std::vector<CT>& h) {
         for (int b = 0; b < 2; ++b)
             HIP_CHECK(hipMemcpyAsync(phib[b], h.data(), h.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
+        init_from_phib();
+    }
+    // The same from phi already in both phi buffers.
+    void init_from_phib() {
         // Distributions, p, u on all nodes of both buffers; outflow planes.
         StepArgs<ST, CT> a = args(1, 1, 1, phib[0], phib[0]);  // fout = pop 0, uout = u[0], prev_out = prev[0]
         const int tb = 256;
@@ -1066,6 +1070,7 @@ struct SolverImpl : Impl {
     CT *lF = nullptr, *lG = nullptr, *lF2 = nullptr, *lG2 = nullptr, *lU = nullptr, *lPhi = nullptr, *lfp = nullptr,
        *lgp = nullptr,
        *lst = nullptr, *lphin = nullptr;
+    double* adj_io = nullptr;  // FP64 staging of the interior arrays (interior nodes)
     CT* lP[2] = {nullptr, nullptr};  // adjoint of the outflow planes (f sides, then g sides), ping-pong
     int lpi = 0;                     // lP[lpi] holds the adjoint of the planes of the state after the step
     double *adj_part = nullptr, *adj_acc = nullptr, *in_accf = nullptr, *in_accg = nullptr;
@@ -1133,12 +1138,29 @@ struct SolverImpl : Impl {
         for (int a = 0; a < L::D; ++a) t *= g.dmax[a] - g.dmin[a] + 1;
         return t;
     }
-    template <class Fn>
-    void for_interior(Fn fn) const {  // fn(interior index, node index), x fastest
-        long long t = 0;
-        for (int k = g.dmin[2]; k <= g.dmax[2]; ++k)
-            for (int j = g.dmin[1]; j <= g.dmax[1]; ++j)
-                for (int i = g.dmin[0]; i <= g.dmax[0]; ++i) fn(t++, g.idx(i, j, k));
+    // Element of interior node (i, j, k) in a host interior array, x fastest
+    // (xslow = 0) or x slowest (xslow = 1); k_adj_from_interior's order.
+    long long interior_index(int i, int j, int k, int xslow) const {
+        const long long n0 = g.dmax[0] - g.dmin[0] + 1, n1 = g.dmax[1] - g.dmin[1] + 1, n2 = g.dmax[2] - g.dmin[2] + 1;
+        const long long li = i - g.dmin[0], lj = j - g.dmin[1], lk = k - g.dmin[2];
+        return xslow ? (li * n1 + lj) * n2 + lk : li + n0 * (lj + n1 * lk);
+    }
+    // Host interior array -> interior nodes of a node array (one transfer
+    // into the FP64 staging buffer, scattered on the device).
+    void interior_in(CT* dst, const double* src, int xslow) {
+        const long long n = interior_count();
+        HIP_CHECK(hipMemcpyAsync(adj_io, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream));
+        k_adj_from_interior<CT><<<(unsigned)((n + ADJ_TB - 1) / ADJ_TB), ADJ_TB, 0, stream>>>(dst, adj_io, g, n, xslow);
+        launch_check("k_adj_from_interior");
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    // a (+ b where non-null) at the interior nodes -> host interior array.
+    void interior_out(double* dst, const CT* a, const CT* b, int xslow) {
+        const long long n = interior_count();
+        k_adj_to_interior<CT><<<(unsigned)((n + ADJ_TB - 1) / ADJ_TB), ADJ_TB, 0, stream>>>(adj_io, a, b, g, n, xslow);
+        launch_check("k_adj_to_interior");
+        HIP_CHECK(hipMemcpyAsync(dst, adj_io, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
     }
     void adj_check() const {
         if constexpr (!std::is_same<ST, CT>::value)
@@ -1297,23 +1319,26 @@ struct SolverImpl : Impl {
         maps_ready = true;
     }
 
-    void adjoint_forward(const std::vector<double>& phi0, int nsteps, int every, std::vector<double>& out) override {
+    void adjoint_forward(const double* phi0, int xslow, int nsteps, int every, double* out) override {
         adj_check();
-        if constexpr (kPlain) adjoint_forward_impl(phi0, nsteps, every, out);
+        if constexpr (kPlain) adjoint_forward_impl(phi0, xslow, nsteps, every, out);
     }
-    void adjoint_forward_impl(const std::vector<double>& phi0, int nsteps, int every, std::vector<double>& out) {
+    void adjoint_forward_impl(const double* phi0, int xslow, int nsteps, int every, double* out) {
         if (nsteps < 1 || every < 1) throw ConfigError("adjoint: nsteps and checkpoint interval must be >= 1");
-        if ((long long)phi0.size() != interior_count()) throw ConfigError("adjoint: phi0 size != interior nodes");
-        // phi on every node: interior from phi0; a ghost cell copies phi0 at
-        // its wrapped position when every axis it lies outside of is periodic,
-        // else it keeps the case's initial phase field (a constant)
+        // phi of the ghost cells: a ghost cell copies phi0 at its wrapped
+        // position when every axis it lies outside of is periodic, else it
+        // keeps the case's initial phase field (a constant); the interior
+        // comes from phi0 on the device (interior_in below)
         std::vector<CT> h((size_t)g.nodes, CT(0));
-        for_interior([&](long long t, long long n) { h[n] = (CT)phi0[t]; });
         for (int k = 0; k < g.ext[2]; ++k)
-            for (int j = 0; j < g.ext[1]; ++j)
+            for (int j = 0; j < g.ext[1]; ++j) {
+                const bool row_in = j >= g.dmin[1] && j <= g.dmax[1] &&
+                                    (L::D == 2 || (k >= g.dmin[2] && k <= g.dmax[2]));
                 for (int i = 0; i < g.ext[0]; ++i) {
-                    const int c[3] = {i, j, k};
-                    if (g.interior(c, L::D)) continue;
+                    if (row_in && i == g.dmin[0]) {
+                        i = g.dmax[0];
+                        continue;
+                    }
                     int w[3] = {i, j, k};
                     bool wrapped = true;
                     for (int a = 0; a < L::D; ++a) {
@@ -1322,8 +1347,10 @@ struct SolverImpl : Impl {
                         const int span = g.dmax[a] - g.dmin[a] + 1;
                         w[a] += (w[a] < g.dmin[a]) ? span : -span;
                     }
-                    h[g.idx(i, j, k)] = wrapped ? h[g.idx(w[0], w[1], w[2])] : (CT)initial_phi(cfg, i, j, k);
+                    h[g.idx(i, j, k)] =
+                        wrapped ? (CT)phi0[interior_index(w[0], w[1], w[2], xslow)] : (CT)initial_phi(cfg, i, j, k);
                 }
+            }
         prev_elems = 0;
         for (int s = 0; s < 6; ++s) {
             poff[s] = prev_elems;
@@ -1355,6 +1382,7 @@ struct SolverImpl : Impl {
             lPhi = adj_alloc_grid<CT>((size_t)g.nodes);
             lphin = adj_alloc_grid<CT>((size_t)g.nodes);
             lst = adj_alloc_grid<CT>((size_t)g.nodes * (L::D + 2));
+            adj_io = adj_alloc<double>((size_t)interior_count());
             if (prev_elems) {
                 lP[0] = adj_alloc<CT>(2 * (size_t)prev_elems);
                 lP[1] = adj_alloc<CT>(2 * (size_t)prev_elems);
@@ -1368,8 +1396,11 @@ struct SolverImpl : Impl {
             in_accf = adj_alloc<double>(std::max<size_t>(1, inlet_rows.size()));
             in_accg = adj_alloc<double>(std::max<size_t>(1, inlet_rows.size()));
         }
-        adj_phi0 = h;
-        init_with_phi(h);
+        HIP_CHECK(hipMemcpyAsync(phib[0], h.data(), h.size() * sizeof(CT), hipMemcpyHostToDevice, stream));
+        interior_in(phib[0], phi0, xslow);
+        HIP_CHECK(hipMemcpyAsync(phib[1], phib[0], h.size() * sizeof(CT), hipMemcpyDeviceToDevice, stream));
+        init_from_phib();
+        adj_phi0 = std::move(h);
         save_state(snap_ptr(0));
         // every other slot starts as a copy of the initial state: cells no
         // step writes (ghost populations, constant ring cells) hold their values
@@ -1380,10 +1411,7 @@ struct SolverImpl : Impl {
             if (n % every == 0 && n < nsteps) save_state(snap_ptr(n / every));
         }
         save_state(final_ptr());
-        std::vector<double> phi;
-        download(phib[phc], (size_t)g.nodes, phi);
-        out.assign((size_t)interior_count(), 0.0);
-        for_interior([&](long long t, long long n) { out[t] = phi[n]; });
+        interior_out(out, phib[phc], nullptr, xslow);
         adj_ready = true;
         adj_grad_ready = false;
         step_count = nsteps;
@@ -1507,15 +1535,12 @@ struct SolverImpl : Impl {
         adj_inner = inner;
     }
 
-    void adjoint_backward(const std::vector<double>& gout, std::vector<double>& gphi0, double* gsigma,
-                          double* gmob) override {
+    void adjoint_backward(const double* gout, int xslow, double* gphi0, double* gsigma, double* gmob) override {
         adj_check();
-        if constexpr (kPlain) adjoint_backward_impl(gout, gphi0, gsigma, gmob);
+        if constexpr (kPlain) adjoint_backward_impl(gout, xslow, gphi0, gsigma, gmob);
     }
-    void adjoint_backward_impl(const std::vector<double>& gout, std::vector<double>& gphi0, double* gsigma,
-                               double* gmob) {
+    void adjoint_backward_impl(const double* gout, int xslow, double* gphi0, double* gsigma, double* gmob) {
         if (!adj_ready) throw ConfigError("adjoint: call the adjoint forward first");
-        if ((long long)gout.size() != interior_count()) throw ConfigError("adjoint: gradient size != interior nodes");
         const size_t nq = (size_t)g.nodes * QL;
         fill(lF, (long long)nq);
         fill(lG, (long long)nq);
@@ -1532,11 +1557,8 @@ struct SolverImpl : Impl {
         HIP_CHECK(hipMemsetAsync(adj_acc, 0, 4 * sizeof(double), stream));
         HIP_CHECK(hipMemsetAsync(in_accf, 0, std::max<size_t>(1, inlet_rows.size()) * sizeof(double), stream));
         HIP_CHECK(hipMemsetAsync(in_accg, 0, std::max<size_t>(1, inlet_rows.size()) * sizeof(double), stream));
-        {
-            std::vector<double> lp((size_t)g.nodes, 0.0);
-            for_interior([&](long long t, long long n) { lp[n] = gout[t]; });
-            upload(lPhi, lp);
-        }
+        fill(lPhi, g.nodes);
+        interior_in(lPhi, gout, xslow);
         for (int k = adj_nsnap - 1; k >= 0; --k) {
             const int s0 = k * adj_K;
             const int s1 = std::min(adj_T, s0 + adj_K);
@@ -1550,23 +1572,18 @@ struct SolverImpl : Impl {
         launch_check("k_adj_init");
         k_adj_accumulate<CT><<<1, ADJ_TB, 0, stream>>>(adj_part, (int)blocks, adj_acc);
         launch_check("k_adj_accumulate");
-        std::vector<double> la, lr;
         if (!shell_nodes()) {
             k_adj_phi_periodic<L, CT, false><<<blocks, ADJ_TB, 0, stream>>>(A, nullptr, 0);
             launch_check("k_adj_phi");
-            download(lphin, (size_t)g.nodes, la);
-            lr.assign((size_t)g.nodes, 0.0);
+            interior_out(gphi0, lphin, nullptr, xslow);
         } else {
             k_adj_phi<L, CT, false><<<nb, ADJ_TB, 0, stream>>>(A, lphin, nullptr, 0);
             launch_check("k_adj_phi");
             fill(lPhi, g.nodes);
             launch_rows(rows_init,
                         [&](unsigned b) { k_adj_ring<CT><<<b, ADJ_TB, 0, stream>>>(rows_init, lphin, lPhi); });
-            download(lphin, (size_t)g.nodes, la);
-            download(lPhi, (size_t)g.nodes, lr);
+            interior_out(gphi0, lphin, lPhi, xslow);
         }
-        gphi0.assign((size_t)interior_count(), 0.0);
-        for_interior([&](long long t, long long n) { gphi0[t] = la[n] + lr[n]; });
         double acc[4];
         HIP_CHECK(hipMemcpyAsync(acc, adj_acc, sizeof(acc), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));

@@ -332,19 +332,28 @@ int clip_write_vts(clip_solver* s, const char* path);
  * together with a wall side, where it acts as one).
  * clip_adjoint_forward builds the initial state of the case
  * (KernelInitializeDistributions, NsAllen.cu:137-202, with the first
- * collision) from phi0 on the interior nodes (x fastest; a ghost cell copies
- * phi0 at its wrapped position when every axis it lies outside of is
- * periodic, else it keeps the case's initial phase field), runs nsteps steps
- * keeping the state (populations, phi with its ghost ring, u, outflow planes)
- * every checkpoint_every steps, and writes phi after the last step (interior,
- * x fastest) to phi_out.
+ * collision) from phi0 on the interior nodes (a ghost cell copies phi0 at its
+ * wrapped position when every axis it lies outside of is periodic, else it
+ * keeps the case's initial phase field), runs nsteps steps keeping the state
+ * (populations, phi with its ghost ring, u, outflow planes) every
+ * checkpoint_every steps, and writes phi after the last step (interior) to
+ * phi_out.
  * clip_adjoint_backward takes dL/dphi_out and returns dL/dphi0 and, if the
  * pointers are non-NULL, dL/dsigma (through beta, kappa and the initial p)
  * and dL/dmobility; it recomputes each segment between kept states once
- * (twice with an inner interval, clip_adjoint_set_nesting). */
-int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out);
+ * (twice with an inner interval, clip_adjoint_set_nesting).
+ * The interior arrays are host FP64 arrays of the interior nodes in the
+ * order `layout` names: CLIP_INTERIOR_X_FASTEST (element i + nx (j + ny k),
+ * i, j, k counted from the first interior node) or CLIP_INTERIOR_X_SLOWEST
+ * (element (i ny + j) nz + k, a C array [nx][ny][nz], or [nx][ny] in 2D).
+ * Each crosses the bus once; the scatter to and gather from the device's
+ * node layout run on the device. */
+#define CLIP_INTERIOR_X_FASTEST 0
+#define CLIP_INTERIOR_X_SLOWEST 1
+int clip_adjoint_forward(clip_solver* s, const double* phi0, int nsteps, int checkpoint_every, double* phi_out,
+                         int layout);
 int clip_adjoint_backward(clip_solver* s, const double* grad_phi_out, double* grad_phi0, double* grad_sigma,
-                          double* grad_mobility);
+                          double* grad_mobility, int layout);
 /* Regularisation of the interface normal's derivative in the reverse sweep
  * (no counterpart in the reference).  0 (default): the exact derivative of
  * the step.  c > 0: where |grad phi| < c 4 phi (1 - phi) / W the derivative

@@ -28,6 +28,21 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(_abi.SIGNATURES)
 
 
+def test_signatures_match_header_prototypes():
+    """Each prototype's parameter count equals its ctypes binding's, and the
+    interior-layout constants agree."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    protos = dict(re.findall(r"\b(clip_[a-z_0-9]+)\s*\(([^)]*)\)\s*;", text))
+    assert len(protos) >= 20
+    for name, params in protos.items():
+        params = params.strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        assert len(_abi.SIGNATURES[name][1]) == n, name
+    defines = dict(re.findall(r"#define\s+(CLIP_INTERIOR_X_[A-Z]+)\s+(\d+)", text))
+    assert int(defines["CLIP_INTERIOR_X_FASTEST"]) == _abi.INTERIOR_X_FASTEST
+    assert int(defines["CLIP_INTERIOR_X_SLOWEST"]) == _abi.INTERIOR_X_SLOWEST
+
+
 def test_ctypes_struct_layout_matches_header():
     src = r"""
 #include <stdio.h>

@@ -247,3 +247,43 @@ def test_adjoint_rejects_16bit_storage_and_lone_slip_sides():
                                            ("y+", "free convect")], n=16, ny=16))
     with pytest.raises(ClipError):
         Solver(slip, "f64").adjoint_forward(np.full((16, 16), 0.5), 2, 1)
+
+
+@pytest.mark.parametrize("text", [cases.drop2d(n=24, radius=6),
+                                  cases.box3d([("x-", "wall"), ("x+", "wall"), ("y-", "periodic"), ("y+", "periodic"),
+                                               ("z-", "free convect"), ("z+", "neumann")], n=(10, 12, 8))],
+                         ids=["drop2d", "box3d_walls"])
+def test_interior_layouts_agree(text):
+    """clip_adjoint_forward / clip_adjoint_backward with x-fastest arrays (the
+    transposes) return the transposes of the x-slowest results, bit for bit,
+    on a non-cubic grid with boundary shells; an unknown layout is an error."""
+    import ctypes
+    from clip_amd import ClipError, _abi
+    cfg = Config.from_string(text)
+    phi0 = Solver(cfg, "f64").init_fresh().interior("phi").reshape(cfg.interior_shape())
+    T, K = 6, 4
+    s = Solver(cfg, "f64")
+    phiT = s.adjoint_forward(phi0, T, K)
+    seed = np.sin(np.arange(phiT.size).reshape(phiT.shape) * 0.37) * 1e-2
+    g0, gs, gm = s.adjoint_backward(seed, 0.1)
+    print("interior layouts:", cfg.interior_shape(), flush=True)
+
+    d = ctypes.POINTER(ctypes.c_double)
+    t = Solver(cfg, "f64")
+    lib = t._lib
+    a = np.ascontiguousarray(phi0.T)
+    out = np.empty_like(a)
+    _abi.check(lib.clip_adjoint_set_nesting(t._h, 0))
+    _abi.check(lib.clip_adjoint_forward(t._h, a.ctypes.data_as(d), T, K, out.ctypes.data_as(d),
+                                        _abi.INTERIOR_X_FASTEST))
+    assert np.array_equal(out.T, phiT)
+    _abi.check(lib.clip_adjoint_set_normal_floor(t._h, 0.1))
+    gin = np.ascontiguousarray(seed.T)
+    gout = np.empty_like(gin)
+    hs, hm = ctypes.c_double(), ctypes.c_double()
+    _abi.check(lib.clip_adjoint_backward(t._h, gin.ctypes.data_as(d), gout.ctypes.data_as(d), ctypes.byref(hs),
+                                         ctypes.byref(hm), _abi.INTERIOR_X_FASTEST))
+    assert np.array_equal(gout.T, g0)
+    assert hs.value == gs and hm.value == gm
+    with pytest.raises(ClipError):
+        _abi.check(lib.clip_adjoint_forward(t._h, a.ctypes.data_as(d), T, K, out.ctypes.data_as(d), 2))
