@@ -267,14 +267,18 @@ def adjoint_measure(precision, n=256, radius=50, steps=500, every=50):
                                   "central_difference_h": "1e-6 x parameter, FP64 forward"}}
 
 
-def adjoint_c5_measure(precision="f32", steps=200, every=20, inner=5):
+def adjoint_c5_measure(precision="f32", steps=200, plans=((20, 5), (14, 0))):
     """Reverse mode on the 3D jet (D3Q19, 192x192x768, modulated disc inlet,
     outflow): loss = 1/2 |phi_T - phi_target|^2 after `steps` steps from the
     case's initial phase field, target from a run with the 8 inlet mode
     amplitudes x 1.5; gradients w.r.t. the 8 amplitudes, sigma and the
-    mobility; states kept every `every` steps and, inside each recomputed
-    segment, every `inner` steps (two levels).  Peak device memory from
-    hipMemGetInfo (through torch) after the checkpointed forward."""
+    mobility.  One entry per checkpoint plan (every, inner): states kept
+    every `every` steps and, with inner > 0, every `inner` steps inside each
+    recomputed segment (two levels) or not (one level: more kept states,
+    one recomputation pass fewer).  Peak device memory from
+    hipMemGetInfo (through torch) after the checkpointed forward; the
+    forward with kept states is timed on a fresh handle (state pool
+    allocation and boundary maps included) and again on the same handle."""
     progress(f"jet3d192x192x768 adjoint {precision}")
     import numpy as np
     import torch
@@ -297,29 +301,44 @@ def adjoint_c5_measure(precision="f32", steps=200, every=20, inner=5):
     base.synchronize()
     fwd = time.perf_counter() - t0
     base.close()
-    torch.cuda.synchronize()
-    free0, total = torch.cuda.mem_get_info()
-    s = Solver(cfg, precision)
-    t0 = time.perf_counter()
-    phiT = s.adjoint_forward(phi0, steps, every, inner_every=inner)
-    t1 = time.perf_counter()
-    free1, _ = torch.cuda.mem_get_info()
-    seed = phiT - target
-    tb = time.perf_counter()
-    g0, gs, gm = s.adjoint_backward(seed, 0.1)
-    t2 = time.perf_counter()
-    ga = s.adjoint_inlet_gradient()
-    held = int(s.info().device_bytes)
-    s.close()
-    return {"workload": "jet3d192x192x768_adjoint", "precision": precision, "lattice": "D3Q19", "steps": steps,
-            "checkpoint_every": every, "inner_checkpoint_every": inner, "normal_floor": 0.1,
-            "forward_ms": round(fwd * 1e3, 3), "forward_with_checkpoints_ms": round((t1 - t0) * 1e3, 3),
-            "backward_ms": round((t2 - tb) * 1e3, 3), "adjoint_over_forward": round((t2 - tb) / fwd, 3),
-            "peak_device_bytes_used": int(total - free1), "device_bytes_before": int(total - free0),
-            "handle_device_bytes": held,
-            "loss": float(0.5 * np.sum((phiT - target) ** 2)),
-            "grad_inlet_amplitudes": [float(x) for x in ga], "grad_sigma": gs, "grad_mobility": gm,
-            "grad_phi0_norm": float(np.linalg.norm(g0))}
+    out, first = [], None
+    for every, inner in plans:
+        progress(f"jet3d192x192x768 adjoint {precision}, kept every {every}, inner {inner}")
+        torch.cuda.synchronize()
+        free0, total = torch.cuda.mem_get_info()
+        s = Solver(cfg, precision)
+        t0 = time.perf_counter()
+        phiT = s.adjoint_forward(phi0, steps, every, inner_every=inner)
+        t1 = time.perf_counter()
+        free1, _ = torch.cuda.mem_get_info()
+        t1a = time.perf_counter()
+        phiT = s.adjoint_forward(phi0, steps, every, inner_every=inner)
+        t1b = time.perf_counter()
+        seed = phiT - target
+        tb = time.perf_counter()
+        g0, gs, gm = s.adjoint_backward(seed, 0.1)
+        t2 = time.perf_counter()
+        ga = s.adjoint_inlet_gradient()
+        held = int(s.info().device_bytes)
+        s.close()
+        e = {"workload": "jet3d192x192x768_adjoint", "precision": precision, "lattice": "D3Q19", "steps": steps,
+             "checkpoint_every": every, "inner_checkpoint_every": inner, "normal_floor": 0.1,
+             "forward_ms": round(fwd * 1e3, 3), "forward_with_checkpoints_ms": round((t1 - t0) * 1e3, 3),
+             "forward_with_checkpoints_again_ms": round((t1b - t1a) * 1e3, 3),
+             "backward_ms": round((t2 - tb) * 1e3, 3), "adjoint_over_forward": round((t2 - tb) / fwd, 3),
+             "peak_device_bytes_used": int(total - free1), "device_bytes_before": int(total - free0),
+             "handle_device_bytes": held,
+             "loss": float(0.5 * np.sum((phiT - target) ** 2)),
+             "grad_inlet_amplitudes": [float(x) for x in ga], "grad_sigma": gs, "grad_mobility": gm,
+             "grad_phi0_norm": float(np.linalg.norm(g0))}
+        if first is None:
+            first = (ga, gs, gm, g0)
+        else:
+            e["gradients_equal_to_first_plan"] = bool(np.array_equal(ga, first[0]) and gs == first[1]
+                                                      and gm == first[2] and np.array_equal(g0, first[3]))
+        del g0, seed, phiT
+        out.append(e)
+    return out
 
 
 def main():
@@ -439,7 +458,7 @@ def main():
                          measure("rti2d", "f64", steps=1000, warmup=20, graph=True),
                          measure("rti2d", "f32", steps=1000, warmup=20, graph=True)]
     if not args.no_adjoint:
-        line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64"), adjoint_c5_measure("f32")]
+        line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64")] + adjoint_c5_measure("f32")
     if not args.no_cpu_baseline:
         progress("CPU baseline (FP64 OpenMP oracle)")
         line["cpu_baseline"] = cpu_baseline(text, args.cpu_steps)
