@@ -69,6 +69,15 @@ struct AdjRows {
     int rows;
 };
 
+// Adjoint of grad phi (dc[2] = 0 in 2D) and of lap phi at one node, one
+// 16-byte (FP32) or 32-byte (FP64) record: the stencil transpose reads a
+// neighbour's record with wide loads instead of D + 1 separate planes.
+template <class CT>
+struct alignas(4 * sizeof(CT)) StAdj {
+    CT dc[3];
+    CT lap;
+};
+
 template <class CT>
 struct AdjArgs {
     DevGrid g;
@@ -86,7 +95,8 @@ struct AdjArgs {
     CT* lPhi;        // in: adjoint of phi^{n+1}; out: adjoint of phi^n (interior)
     CT* lfp;         // scratch Q x nodes: adjoint of f_post
     CT* lgp;         // scratch Q x nodes: adjoint of g_post
-    CT* lst;         // scratch (D+2) x nodes: adjoint of phi, grad phi, lap phi at the node
+    CT* lst;         // scratch, nodes: adjoint of phi at the node (the stencil's centre value)
+    StAdj<CT>* lsv;  // scratch, nodes: adjoint of grad phi and lap phi at the node
     CT* lphin;       // scratch nodes: adjoint of phi^{n+1}
     double* part;    // 4 x blocks partial sums: beta, kappa, mobility, sigma
     CT sigma, half_length;
@@ -267,12 +277,9 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_node(AdjArgs<CT> a) {
             }
         }
         a.lst[n] = o.lc0;
+        a.lsv[n] = StAdj<CT>{{o.ldc[0], o.ldc[1], D == 3 ? o.ldc[2] : CT(0)}, o.llap};
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            a.lst[(long long)(1 + d) * g.nodes + n] = o.ldc[d];
-            a.lU[(long long)d * g.nodes + n] = o.luo[d];
-        }
-        a.lst[(long long)(1 + D) * g.nodes + n] = o.llap;
+        for (int d = 0; d < D; ++d) a.lU[(long long)d * g.nodes + n] = o.luo[d];
         pb = o.lbeta;
         pk = o.lkappa;
         pm = o.lmob;
@@ -366,10 +373,10 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi(AdjArgs<CT> a, CT* lall, dou
                 if (!g.interior(z, D)) continue;
                 CT wg[3], wl;
                 stencil_weights<D, CT>(dx, dy, dz, wg, wl);
-                const long long m = g.idx(z[0], z[1], z[2]);
-                CT v = wl * a.lst[(long long)(1 + D) * g.nodes + m];
+                const StAdj<CT> r = a.lsv[g.idx(z[0], z[1], z[2])];
+                CT v = wl * r.lap;
 #pragma unroll
-                for (int d = 0; d < D; ++d) v += wg[d] * a.lst[(long long)(1 + d) * g.nodes + m];
+                for (int d = 0; d < D; ++d) v += wg[d] * r.dc[d];
                 total += v;
             }
     lall[y] = total;
@@ -407,10 +414,10 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_phi_periodic(AdjArgs<CT> a, doub
                 const bool zin = g.interior(z, D);
                 const int zx = wrap_axis(g, 0, z[0]), zy = wrap_axis(g, 1, z[1]);
                 const int zz = (D == 3) ? wrap_axis(g, 2, z[2]) : z[2];
-                const long long m = g.idx(zx, zy, zz);
-                CT v = wl * a.lst[(long long)(1 + D) * g.nodes + m];
+                const StAdj<CT> r = a.lsv[g.idx(zx, zy, zz)];
+                CT v = wl * r.lap;
 #pragma unroll
-                for (int d = 0; d < D; ++d) v += wg[d] * a.lst[(long long)(1 + d) * g.nodes + m];
+                for (int d = 0; d < D; ++d) v += wg[d] * r.dc[d];
                 if (zin || !LAG) total += v;
                 else lag += v;
             }
@@ -594,9 +601,7 @@ __global__ __launch_bounds__(ADJ_TB) void k_adj_init(AdjArgs<CT> a) {
         init_reverse<L, CT>(a.ph, a.sigma, a.half_length, c0, dc, lap, lfs, lgs, lc0, ldc, llap, lb, lk, lm, lsg,
                             a.nfloor, lpf, lpg);
         a.lst[n] = lc0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) a.lst[(long long)(1 + d) * g.nodes + n] = ldc[d];
-        a.lst[(long long)(1 + D) * g.nodes + n] = llap;
+        a.lsv[n] = StAdj<CT>{{ldc[0], ldc[1], D == 3 ? ldc[2] : CT(0)}, llap};
         pb = lb;
         pk = lk;
         pm = lm;

@@ -1070,6 +1070,7 @@ struct SolverImpl : Impl {
     CT *lF = nullptr, *lG = nullptr, *lF2 = nullptr, *lG2 = nullptr, *lU = nullptr, *lPhi = nullptr, *lfp = nullptr,
        *lgp = nullptr,
        *lst = nullptr, *lphin = nullptr;
+    StAdj<CT>* lsv = nullptr;  // adjoint of grad phi, lap phi per node (k_adj_node -> k_adj_phi)
     double* adj_io = nullptr;  // FP64 staging of the interior arrays (interior nodes)
     CT* lP[2] = {nullptr, nullptr};  // adjoint of the outflow planes (f sides, then g sides), ping-pong
     int lpi = 0;                     // lP[lpi] holds the adjoint of the planes of the state after the step
@@ -1381,7 +1382,10 @@ struct SolverImpl : Impl {
             lU = adj_alloc_grid<CT>((size_t)g.nodes * L::D);
             lPhi = adj_alloc_grid<CT>((size_t)g.nodes);
             lphin = adj_alloc_grid<CT>((size_t)g.nodes);
-            lst = adj_alloc_grid<CT>((size_t)g.nodes * (L::D + 2));
+            lst = adj_alloc_grid<CT>((size_t)g.nodes);
+            // records of 4 CT, 16-byte aligned or more; node x = 1 of a row on a
+            // 128-byte line (the row stride is a multiple of 128 bytes of CT)
+            lsv = adj_alloc<StAdj<CT>>((size_t)g.nodes + 8) + (row_align ? 128 / sizeof(StAdj<CT>) - 1 : 0);
             adj_io = adj_alloc<double>((size_t)interior_count());
             if (prev_elems) {
                 lP[0] = adj_alloc<CT>(2 * (size_t)prev_elems);
@@ -1437,6 +1441,7 @@ struct SolverImpl : Impl {
         A.lfp = lfp;
         A.lgp = lgp;
         A.lst = lst;
+        A.lsv = lsv;
         A.lphin = lphin;
         A.part = adj_part;
         A.sigma = (CT)cfg.sigma;
@@ -1547,7 +1552,8 @@ struct SolverImpl : Impl {
         fill(lF2, (long long)nq);
         fill(lG2, (long long)nq);
         fill(lU, g.nodes * L::D);
-        fill(lst, g.nodes * (L::D + 2));
+        fill(lst, g.nodes);
+        fill((CT*)lsv, g.nodes * 4);
         fill(lphin, g.nodes);
         lpi = 0;
         if (prev_elems) {
