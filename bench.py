@@ -64,14 +64,28 @@ def load_traffic(kernel, workload, precision):
 
 
 def cpu_model():
+    """The host CPU's model name with its family / model numbers (a virtual
+    machine may report only a generic name) and the cores this process may use."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if fields:
+                        break  # first processor only
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                fields.setdefault(k, v)
     except OSError:
+        return "unknown"
+    name = fields.get("model name", "unknown")
+    if "cpu family" in fields and "model" in fields:
+        name += f" (cpu family {fields['cpu family']}, model {fields['model']})"
+    try:
+        name += f", {len(os.sched_getaffinity(0))} cores available"
+    except AttributeError:
         pass
-    return "unknown"
+    return name
 
 
 def cpu_baseline(text, steps, runs=3):
