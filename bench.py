@@ -469,7 +469,9 @@ def main():
                          measure("jet3d192x192x768", "f64"),
                          measure("drop2d128", "f64", steps=1000, warmup=20),
                          measure("drop2d128", "f64", steps=1000, warmup=20, graph=True),
+                         measure("rti2d", "f64", steps=1000, warmup=20),
                          measure("rti2d", "f64", steps=1000, warmup=20, graph=True),
+                         measure("rti2d", "f32", steps=1000, warmup=20),
                          measure("rti2d", "f32", steps=1000, warmup=20, graph=True)]
     if not args.no_adjoint:
         line["adjoint"] = [adjoint_measure("f32"), adjoint_measure("f64")] + adjoint_c5_measure("f32")

@@ -744,9 +744,37 @@ __device__ __forceinline__ bool select_node(const DevGrid& g, const DevBC& bc, c
 }
 
 // ------------------------------------------------------------ step kernels
+// The phi ghost refresh of a step without inlet sides (one k_ghost_resolve
+// pass, periodic and mirror copies from the old phi) carried by extra blocks
+// of the phase launch: it writes only the new phi's ghost ring and reads only
+// the old phi, while the phase sum writes the new phi's interior from g, so
+// one launch does both.  Blocks from `first` on (the z index of the box grid
+// in k_phase, the block index in k_phase_pair) visit the ghost cells.
+template <class CT>
+__device__ inline void ghost_resolve_cell(const CT* oldp, CT* newp, const DevGrid& g, const DevBC& bc, int D,
+                                          long long t, int do_periodic, int do_mirror, int ghost_from_new);
+template <class CT>
+struct GhostJob {
+    const CT* oldp;
+    CT* newp;
+    long long total;  // ghost cells; 0: no ghost blocks in this launch
+    unsigned first;
+};
+template <class CT>
+__device__ __forceinline__ void ghost_job_block(const GhostJob<CT>& gj, const DevGrid& g, const DevBC& bc, int D,
+                                                long long block) {
+    const long long t = block * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    if (t < gj.total) ghost_resolve_cell(gj.oldp, gj.newp, g, bc, D, t, 1, 1, 0);
+}
+
 // phi^{n+1} = sum_q g_post (kernelMacroscopicg, NsAllen.cu:516-540).
 template <class L, class ST, class CT, class SP, bool DIRECT, bool SHELL, bool ROWS = false>
-__global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a) {
+__global__ __launch_bounds__(BX* BY) void k_phase(StepArgs<ST, CT> a, GhostJob<CT> gj) {
+    if (gj.total && blockIdx.z >= gj.first) {
+        ghost_job_block(gj, a.g, a.bc, L::D,
+                        ((long long)(blockIdx.z - gj.first) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
+        return;
+    }
     int c[3];
     if (!select_node<L, DIRECT, SHELL, ROWS>(a.g, a.bc, a.shell, c)) return;
     const long long n = a.g.idx(c[0], c[1], c[2]);
@@ -1180,7 +1208,11 @@ __device__ __forceinline__ long long pair_row_offset(const PullOffsets& po, int 
 }
 
 template <class L, class ST, class SP>
-__global__ __launch_bounds__(BX* BY) void k_phase_pair(StepArgs<ST, float> a, PairGrid pg) {
+__global__ __launch_bounds__(BX* BY) void k_phase_pair(StepArgs<ST, float> a, PairGrid pg, GhostJob<float> gj) {
+    if (gj.total && blockIdx.x >= gj.first) {
+        ghost_job_block(gj, a.g, a.bc, L::D, (long long)(blockIdx.x - gj.first));
+        return;
+    }
     int c[3];
     bool has1;
     if (!pair_of_thread(pg, c, has1)) return;
@@ -1238,10 +1270,8 @@ __device__ inline CT ghost_source(const CT* oldp, const CT* newp, const DevGrid&
 // axis wrapped and every mirror axis clamped.  The source cell is never
 // written by the same launch.
 template <class CT>
-__global__ void k_ghost_resolve(const CT* oldp, CT* newp, DevGrid g, DevBC bc, int D, long long total,
-                                int do_periodic, int do_mirror, int ghost_from_new) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
+__device__ inline void ghost_resolve_cell(const CT* oldp, CT* newp, const DevGrid& g, const DevBC& bc, int D,
+                                          long long t, int do_periodic, int do_mirror, int ghost_from_new) {
     int c[3];
     if (!ghost_cell_of_thread(g, D, t, c)) return;
     int s[3] = {c[0], c[1], c[2]};
@@ -1263,6 +1293,12 @@ __global__ void k_ghost_resolve(const CT* oldp, CT* newp, DevGrid g, DevBC bc, i
         }
     }
     if (changed) newp[g.idx(c[0], c[1], c[2])] = ghost_source(oldp, newp, g, D, s, ghost_from_new);
+}
+template <class CT>
+__global__ void k_ghost_resolve(const CT* oldp, CT* newp, DevGrid g, DevBC bc, int D, long long total,
+                                int do_periodic, int do_mirror, int ghost_from_new) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < total) ghost_resolve_cell(oldp, newp, g, bc, D, t, do_periodic, do_mirror, ghost_from_new);
 }
 
 // Inlet side, outside the disc: phi ghost <- boundary node (Solver.cu:621-633).

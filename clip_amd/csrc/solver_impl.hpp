@@ -331,20 +331,52 @@ struct SolverImpl : Impl {
     }
     // phi^{n+1} of the bulk (k_phase) and of the boundary layer (k_shell_pull,
     // which also leaves the layer's f_post / g_post for k_fused).
-    void launch_phase_pull(const StepArgs<ST, CT>& a) {
+    // The bulk phase launch.  A ghost job (gj.total > 0, from ghost_job) rides
+    // on it as extra blocks, or runs as its own k_ghost_resolve launch when
+    // there is no bulk launch.
+    void launch_phase_pull(const StepArgs<ST, CT>& a, GhostJob<CT> gj = GhostJob<CT>{}) {
         const dim3 gb = grid_box(1);
-        bool done = false;
+        bool done = false, ghost_done = gj.total == 0;
+        // the box grid with enough extra z-slices of blocks for the ghost cells
+        auto with_ghosts = [&](dim3 grid) {
+            if (!gj.total) return grid;
+            const unsigned long long tpb = (unsigned long long)tbx * tby, per_slice = (unsigned long long)grid.x * grid.y;
+            const unsigned long long blocks = ((unsigned long long)gj.total + tpb - 1) / tpb;
+            gj.first = grid.z;
+            grid.z += (unsigned)((blocks + per_slice - 1) / per_slice);
+            ghost_done = true;
+            return grid;
+        };
         if constexpr (kRowAddressing) {
             done = gb.x && tby == 1;
-            if (done) k_phase<L, ST, CT, SP, false, false, true><<<gb, block_shape(), 0, stream>>>(a);
+            if (done) {
+                const dim3 grid = with_ghosts(gb);
+                k_phase<L, ST, CT, SP, false, false, true><<<grid, block_shape(), 0, stream>>>(a, gj);
+            }
         }
         if constexpr (kPairs) {
             const PairGrid pg = pair_grid();
-            if (pg.total) k_phase_pair<L, ST, SP><<<pair_blocks(pg), BX * BY, 0, stream>>>(a, pg);
+            if (pg.total) {
+                unsigned blocks = pair_blocks(pg);
+                if (gj.total) {
+                    gj.first = blocks;
+                    blocks += (unsigned)((gj.total + BX * BY - 1) / (BX * BY));
+                    ghost_done = true;
+                }
+                k_phase_pair<L, ST, SP><<<blocks, BX * BY, 0, stream>>>(a, pg, gj);
+            }
             done = true;
         }
-        if (gb.x && !done) k_phase<L, ST, CT, SP, false, false><<<gb, block_shape(), 0, stream>>>(a);
+        if (gb.x && !done) {
+            const dim3 grid = with_ghosts(gb);
+            k_phase<L, ST, CT, SP, false, false><<<grid, block_shape(), 0, stream>>>(a, gj);
+        }
         launch_check("k_phase");
+        if (!ghost_done) {
+            k_ghost_resolve<CT><<<(unsigned)((gj.total + 255) / 256), 256, 0, stream>>>(gj.oldp, gj.newp, g, bc, L::D,
+                                                                                       gj.total, 1, 1, 0);
+            launch_check("k_ghost_resolve");
+        }
         if (shell_blocks()) {
             launch_shell_pull(a);
             k_shell_phi<L, ST, CT><<<shell_blocks(), BX * BY, 0, stream>>>(a);
@@ -356,6 +388,14 @@ struct SolverImpl : Impl {
         launch_check("k_shell_pull");
     }
 
+    // Without inlet sides the ghost ring is one k_ghost_resolve pass, which
+    // can ride on the phase launch (launch_phase_pull); total = 0 otherwise.
+    GhostJob<CT> ghost_job(const CT* phi_old, CT* phi_new) const {
+        if (bc.is_velocity) return GhostJob<CT>{};
+        long long total = 0;
+        for (int s = 0; s < 2 * L::D; ++s) total += plane_size(g, s);
+        return GhostJob<CT>{phi_old, phi_new, total, 0u};
+    }
     // Ghost ring of phi_new from phi_old (Solver.cu:75-141, 550-921, 927-978).
     void ghost_refresh(const CT* phi_old, CT* phi_new) {
         long long total = 0;
@@ -472,9 +512,11 @@ struct SolverImpl : Impl {
         StepArgs<ST, CT> a = args(cur, ucur, pcur, phi_new, phi_old);
         const bool tm = timing_on && !capturing;
         if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
-        ghost_refresh(phi_old, phi_new);
+        // the ghost ring: with the phase launch when it is one pass
+        const GhostJob<CT> gj = ghost_job(phi_old, phi_new);
+        if (!gj.total) ghost_refresh(phi_old, phi_new);
         if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
-        launch_phase_pull(a);
+        launch_phase_pull(a, gj);
         if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
         launch_fused_pull<MODE_FULL>(a);
         if (tm) {
@@ -835,7 +877,7 @@ struct SolverImpl : Impl {
                                  stream));
         // pull buffer 0 in place, u[ucur] -> u[1-ucur], outflow planes -> prev[pcur]
         StepArgs<ST, CT> a = args(0, ucur, 1 - pcur, phib[1 - phc], nullptr);
-        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a, GhostJob<CT>{});
         launch_check("k_phase");
         k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
         launch_check("k_fused");
@@ -1035,7 +1077,7 @@ struct SolverImpl : Impl {
         std::vector<double> zero(hn() * L::D, 0.0);
         upload_field(u[0], zero, L::D);
         StepArgs<ST, CT> a = args(0, 0, 1, phib[1], nullptr);
-        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+        k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a, GhostJob<CT>{});
         launch_check("k_phase");
         // ghost ring from the resumed phi itself
         HIP_CHECK(hipMemcpyAsync(phib[0], phib[1], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice, stream));
@@ -1129,8 +1171,9 @@ struct SolverImpl : Impl {
                 s_.prev_f_out[s] = fc ? st_pf(b) + poff[s] : nullptr;
                 s_.prev_g_out[s] = fc ? st_pg(b) + poff[s] : nullptr;
             }
-            ghost_refresh(st_phi(a), st_phi(b));
-            launch_phase_pull(s_);
+            const GhostJob<CT> gj = ghost_job(st_phi(a), st_phi(b));
+            if (!gj.total) ghost_refresh(st_phi(a), st_phi(b));
+            launch_phase_pull(s_, gj);
             launch_fused_pull<MODE_FULL>(s_);
         }
     }
