@@ -922,13 +922,23 @@ enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 // normal_FD 241-275, kernelCollisionMRTg 545-584, kernelCollisionMRTf 589-763).
 // FP64: the reference's transforms and operation order.  FP32 / mixed: the
 // exact moment-space form of the same step (node.hpp).
-template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS = false>
-__global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
+// The node of this thread and its update.  CARRIED: a boundary-layer node
+// taken by the extra z-slices (from shell_first on) of a bulk launch.
+template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS, bool CARRIED>
+__device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned shell_first) {
     constexpr int Q = L::Q;
     constexpr bool MOMENT_FORM = sizeof(CT) == 4;
     int c[3];
-    if (!select_node<L, DIRECT, SHELL, ROWS>(a.g, a.bc, a.shell, c)) return;
-    const long long shell_t = SHELL ? (long long)blockIdx.x * blockDim.x + threadIdx.x : 0;
+    long long shell_t = 0;
+    if constexpr (CARRIED) {
+        const long long block =
+            ((long long)(blockIdx.z - shell_first) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        shell_t = block * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+        if (!shell_node_at<L>(a.g, a.shell, shell_t, c)) return;
+    } else {
+        if (!select_node<L, DIRECT, SHELL, ROWS>(a.g, a.bc, a.shell, c)) return;
+        if (SHELL) shell_t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    }
     const DevGrid& g = a.g;
     const Phys<CT>& ph = a.ph;
     const long long n = g.idx(c[0], c[1], c[2]);
@@ -1118,6 +1128,23 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) SP::st(dst_ptr(a.fout, q), 0, q, fp[q] - gneq[q] + hlp[q], CT(0));
     }
+}
+
+// The launch over the bulk box (or over the boundary layer, SHELL).  A bulk
+// launch may carry the boundary layer too (CARRY): blocks from z index
+// shell_first on take its nodes, which the bulk nodes neither read nor write
+// in this kernel, so one launch collides both.  The carrying instantiation
+// holds both code paths and the registers of the larger one.
+template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS = false,
+          bool CARRY = false>
+__global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a, unsigned shell_first) {
+    if constexpr (CARRY && !SHELL && !DIRECT) {
+        if (blockIdx.z >= shell_first) {
+            fused_node<L, ST, CT, SP, DIRECT, MODE, true, false, true>(a, shell_first);
+            return;
+        }
+    }
+    fused_node<L, ST, CT, SP, DIRECT, MODE, SHELL, ROWS, false>(a, 0u);
 }
 
 // ------------------------------------------------------------ node pairs

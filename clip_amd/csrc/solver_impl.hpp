@@ -320,13 +320,34 @@ struct SolverImpl : Impl {
     template <int MODE>
     void launch_fused_pull(const StepArgs<ST, CT>& a) {
         const dim3 gb = grid_box(1);
+        const long long S = shell_nodes();
+        if constexpr (kCarryShell) {
+            if (gb.x && S) {
+                // the boundary layer's nodes ride on the bulk launch as extra z-slices
+                dim3 grid = gb;
+                const unsigned long long tpb = (unsigned long long)tbx * tby,
+                                         per_slice = (unsigned long long)gb.x * gb.y;
+                const unsigned long long blocks = ((unsigned long long)S + tpb - 1) / tpb;
+                grid.z += (unsigned)((blocks + per_slice - 1) / per_slice);
+                bool rows = false;
+                if constexpr (kRowAddressing) {
+                    rows = tby == 1;
+                    if (rows)
+                        k_fused<L, ST, CT, SP, false, MODE, false, true, true><<<grid, block_shape(), 0, stream>>>(a, gb.z);
+                }
+                if (!rows)
+                    k_fused<L, ST, CT, SP, false, MODE, false, false, true><<<grid, block_shape(), 0, stream>>>(a, gb.z);
+                launch_check("k_fused");
+                return;
+            }
+        }
         bool done = false;
         if constexpr (kRowAddressing) {
             done = gb.x && tby == 1;
-            if (done) k_fused<L, ST, CT, SP, false, MODE, false, true><<<gb, block_shape(), 0, stream>>>(a);
+            if (done) k_fused<L, ST, CT, SP, false, MODE, false, true><<<gb, block_shape(), 0, stream>>>(a, 0u);
         }
-        if (gb.x && !done) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a);
-        if (shell_blocks()) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a);
+        if (gb.x && !done) k_fused<L, ST, CT, SP, false, MODE, false><<<gb, block_shape(), 0, stream>>>(a, 0u);
+        if (S) k_fused<L, ST, CT, SP, false, MODE, true><<<shell_blocks(), BX * BY, 0, stream>>>(a, 0u);
         launch_check("k_fused");
     }
     // phi^{n+1} of the bulk (k_phase) and of the boundary layer (k_shell_pull,
@@ -489,7 +510,7 @@ struct SolverImpl : Impl {
         inlet_fill(phib[0]);
         // First collision straight from the initial f_post = f (pop 0).
         StepArgs<ST, CT> c = args(0, 0, 1, phib[0], phib[0]);
-        k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(c);
+        k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(c, 0u);
         launch_check("k_fused");
         HIP_CHECK(hipStreamSynchronize(stream));
         cur = 1;
@@ -879,7 +900,7 @@ struct SolverImpl : Impl {
         StepArgs<ST, CT> a = args(0, ucur, 1 - pcur, phib[1 - phc], nullptr);
         k_phase<L, ST, CT, SP, true, false><<<grid_interior(), block_shape(), 0, stream>>>(a, GhostJob<CT>{});
         launch_check("k_phase");
-        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a, 0u);
         launch_check("k_fused");
         const int tb = 256;
         k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(a, true, scratch, p);
@@ -942,7 +963,7 @@ struct SolverImpl : Impl {
         a.ucol = u[ucur];
         a.uout = nullptr;
         if (direct_old) {
-            k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+            k_fused<L, ST, CT, SP, true, MODE_COLLIDE, false><<<grid_interior(), block_shape(), 0, stream>>>(a, 0u);
             launch_check("k_fused");
         } else {
             if (shell_blocks()) launch_shell_pull(a);
@@ -1082,7 +1103,7 @@ struct SolverImpl : Impl {
         // ghost ring from the resumed phi itself
         HIP_CHECK(hipMemcpyAsync(phib[0], phib[1], (size_t)g.nodes * sizeof(CT), hipMemcpyDeviceToDevice, stream));
         ghost_refresh(phib[0], phib[1]);
-        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a);
+        k_fused<L, ST, CT, SP, true, MODE_FULL, false><<<grid_interior(), block_shape(), 0, stream>>>(a, 0u);
         launch_check("k_fused");
         k_export_post<L, ST, CT, SP, true><<<(unsigned)((g.nodes + tb - 1) / tb), tb, 0, stream>>>(a, true, scratch, p);
         launch_check("k_export_post");
@@ -1259,6 +1280,10 @@ struct SolverImpl : Impl {
     }
 
     static constexpr bool kPlain = std::is_same<ST, CT>::value;
+    // Whether the bulk collision launch carries the boundary layer: not for
+    // FP64 D3Q19, whose boundary-layer path needs 222 VGPRs against the
+    // bulk's 164 (the carrying kernel would drop from 3 to 2 waves per SIMD).
+    static constexpr bool kCarryShell = !(std::is_same<CT, double>::value && L::Q == 19);
 
     bool shell_host(const int c[3]) const {
         bool s = false;
