@@ -63,6 +63,11 @@ def load_traffic(kernel, workload, precision):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+# The cores this process may use, read before any OpenMP region runs (the
+# oracle's OMP_PROC_BIND=close binds the calling thread to one core).
+HOST_CORES = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
 def cpu_model():
     """The host CPU's model name with its family / model numbers (a virtual
     machine may report only a generic name) and the cores this process may use."""
@@ -81,11 +86,7 @@ def cpu_model():
     name = fields.get("model name", "unknown")
     if "cpu family" in fields and "model" in fields:
         name += f" (cpu family {fields['cpu family']}, model {fields['model']})"
-    try:
-        name += f", {len(os.sched_getaffinity(0))} cores available"
-    except AttributeError:
-        pass
-    return name
+    return name + f", {HOST_CORES} cores available"
 
 
 def cpu_baseline(text, steps, runs=3):
