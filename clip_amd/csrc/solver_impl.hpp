@@ -1474,9 +1474,11 @@ struct SolverImpl : Impl {
         init_from_phib();
         adj_phi0 = std::move(h);
         save_state(snap_ptr(0));
-        // every other slot starts as a copy of the initial state: cells no
-        // step writes (ghost populations, constant ring cells) hold their values
-        for (size_t k = 1; k < need; ++k)
+        // the recomputation slots (inner level, leaves) start as a copy of the
+        // initial state: cells no step writes (ghost populations, constant
+        // ring cells) hold their values.  The kept states and the final one
+        // are whole copies of the handle's state (save_state).
+        for (size_t k = (size_t)adj_nsnap + 1; k < need; ++k)
             HIP_CHECK(hipMemcpyAsync(slot(k), slot(0), state_elems() * sizeof(CT), hipMemcpyDeviceToDevice, stream));
         for (int n = 1; n <= nsteps; ++n) {
             step(1);
