@@ -238,3 +238,40 @@ def test_mixed_against_fp64_oracle(precision, tol_phi, tol_u, tol_p, name, text,
     assert e_phi <= tol_phi, e_phi
     assert e_u <= tol_u, e_u
     assert e_p <= tol_p, e_p
+
+
+# Domains so small that the bulk box (the interior without the boundary
+# layer) is one node wide, the narrowest the library accepts (3 interior
+# nodes per axis); the phi ghost ring and the boundary layer's collision ride
+# on bulk launches of a single node per row.  Two interior nodes on an axis
+# are rejected.
+TINY = {
+    "box2d_3x9": cases.box2d([("x-", "neumann"), ("x+", "wall"), ("y-", "wall"), ("y+", "do nothing")],
+                             n=3, ny=9, radius=1.5),
+    "box3d_3x3x4": cases.box3d([("x-", "wall"), ("x+", "slip wall"), ("y-", "wall"), ("y+", "wall"),
+                                ("z-", "wall"), ("z+", "wall")], n=(3, 3, 4), radius=1.0),
+    "box3d_3x6x5": cases.box3d([("x-", "wall"), ("x+", "wall"), ("y-", "periodic"), ("y+", "periodic"),
+                                ("z-", "free convect"), ("z+", "neumann")], n=(3, 6, 5), radius=1.0),
+}
+
+
+@pytest.mark.parametrize("precision,tol_phi", [("f64", TOL), ("f32", 2e-4), ("mixed_f16", 2e-3)])
+@pytest.mark.parametrize("name", sorted(TINY))
+def test_tiny_bounded_domains(name, precision, tol_phi):
+    cfg, gpu, orc = run_pair(TINY[name], 20, precision=precision)
+    if precision == "f64":
+        assert_fields(gpu, orc)
+        return
+    a = gpu.interior("phi")
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(gpu.interior("u")))
+    e = rel_l2(a, orc.interior("phi"))
+    print(name, precision, f"phi rel-L2 {e:.2e}", flush=True)
+    assert e <= tol_phi, e
+
+
+def test_two_node_interior_is_rejected():
+    from clip_amd import ClipError
+    cfg = Config.from_string(cases.box2d([("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "wall")],
+                                         n=2, ny=8, radius=1.0))
+    with pytest.raises(ClipError, match="at least 3 nodes"):
+        Solver(cfg, "f64")
