@@ -191,7 +191,10 @@ def test_sum_matches_interior_sum():
     ("drop2d", cases.drop2d(n=32, radius=8), 100, 2e-4),
     ("rti2d", cases.rti2d(32, 64), 200, 2e-4),
     ("bubble3d", cases.bubble3d(n=16, radius=4), 60, 2e-4),
-], ids=["drop2d", "rti2d", "bubble3d"])
+    ("box3d_walls_15", cases.bubble3d(n=15, radius=3, rho_ratio=10, mu_ratio=10,
+                                      sides=[("x-", "wall"), ("x+", "wall"), ("y-", "wall"), ("y+", "free convect"),
+                                             ("z-", "periodic"), ("z+", "periodic")]), 60, 2e-4),
+], ids=["drop2d", "rti2d", "bubble3d", "box3d_walls_15"])
 def test_fp32_against_fp64_oracle(name, text, steps, tol):
     cfg, gpu, orc = run_pair(text, steps, precision="f32")
     e_phi = rel_l2(gpu.interior("phi"), orc.interior("phi"))
