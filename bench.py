@@ -50,6 +50,12 @@ def workload_text(name):
         return cases.drop2d(n=128, radius=25)
     if name == "jet3d192x192x768":
         return cases.jet3d_c5()
+    if name == "slab192x192x96":
+        # one rank's share of the jet split eight ways along z: a fully
+        # periodic box of that shape, whose z ghost planes (refreshed by the
+        # periodic wrap) stand in for the halo planes a rank would receive
+        return cases.box3d([(s, "periodic") for s in ("x-", "x+", "y-", "y+", "z-", "z+")], n=(192, 192, 96),
+                           radius=16.0)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -468,6 +474,8 @@ def main():
         line["extra"] = [measure(args.workload, other, against_f64=True),
                          measure("jet3d192x192x768", "mixed_f16", against_f64=True),
                          measure("jet3d192x192x768", "f64"),
+                         measure("slab192x192x96", "mixed_f16"),
+                         measure("slab192x192x96", "f64"),
                          measure("drop2d128", "f64", steps=1000, warmup=20),
                          measure("drop2d128", "f64", steps=1000, warmup=20, graph=True),
                          measure("rti2d", "f64", steps=1000, warmup=20),

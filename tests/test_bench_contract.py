@@ -28,6 +28,7 @@ def test_more_than_one_gpu_is_refused_before_any_gpu_work():
     ("jet3d192x192x768", 3, (192, 192, 768)),
     ("drop2d128", 2, (128, 128)),
     ("rti2d", 2, (256, 1024)),
+    ("slab192x192x96", 3, (192, 192, 96)),
 ])
 def test_named_workloads_have_the_quoted_sizes(name, dim, interior):
     cfg = Config.from_string(bench.workload_text(name))
