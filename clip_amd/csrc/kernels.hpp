@@ -923,9 +923,15 @@ enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 // FP64: the reference's transforms and operation order.  FP32 / mixed: the
 // exact moment-space form of the same step (node.hpp).
 // The node of this thread and its update.  CARRIED: a boundary-layer node
-// taken by the extra z-slices (from shell_first on) of a bulk launch.
-template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS, bool CARRIED>
-__device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned shell_first) {
+// taken by the extra z-slices (from shell_first on) of a bulk launch.  TILE:
+// the phi stencil reads `tile`, the block's phi^{n+1} with a one-node halo in
+// LDS, rows of blockDim.x + 2 (k_step_tile), instead of the phi array, and
+// the pulled f and g are the caller's registers pre_f / pre_g.
+template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS, bool CARRIED,
+          bool TILE = false>
+__device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned shell_first,
+                                           const CT* tile = nullptr, const CT* pre_f = nullptr,
+                                           const CT* pre_g = nullptr) {
     constexpr int Q = L::Q;
     constexpr bool MOMENT_FORM = sizeof(CT) == 4;
     int c[3];
@@ -948,7 +954,14 @@ __device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned s
     // phi stencil -> grad phi, chemical potential
     CT c0, dc[3], lap;
     CT nbq[Q];  // phi at n + e_q (the stencil's own loads)
-    phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap, SP::uses_ref ? nbq : nullptr);
+    if constexpr (TILE) {
+        static_assert(L::D == 2 && !SP::uses_ref, "the LDS tile serves the 2D plain-storage step");
+        DevGrid tg = g;
+        tg.sy = blockDim.x + 2;
+        phi_stencil<L, CT>(tile, tg, (long long)(threadIdx.y + 1) * tg.sy + threadIdx.x + 1, c0, dc, lap, nullptr);
+    } else {
+        phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap, SP::uses_ref ? nbq : nullptr);
+    }
     // reference of the stored g (mixed policy): phi of the node that pulls it
     CT refq[SP::uses_ref ? Q : 1];
     if constexpr (SP::uses_ref) g_store_refs<L>(a, c, n, nbq, refq);
@@ -978,7 +991,7 @@ __device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned s
         for (int q = 0; q < Q; ++q) {
             if (DIRECT) fp[q] = SP::ld(a.fin, (long long)q * g.nodes + n, q, CT(0));
             else if (!SHELL)
-                fp[q] = SP::ld(src_ptr(a.fin, q), 0, q, CT(0));
+                fp[q] = TILE ? pre_f[q] : SP::ld(src_ptr(a.fin, q), 0, q, CT(0));
             else
                 fp[q] = a.shell_f[(long long)q * shell_total(a.shell) + shell_t];
         }
@@ -991,6 +1004,7 @@ __device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned s
     const CT rn = (!DIRECT && !SHELL) ? gref<SP>(a.phi_ref, n) : CT(0);
     auto pull_g = [&](int q) -> CT {
         if (DIRECT) return SP::ld(a.gin, (long long)q * g.nodes + n, q, gref<SP>(a.phi_ref, ref_index<L>(g, a.bc, c, q)));
+        if (TILE) return pre_g[q];
         if (!SHELL) return SP::ld(src_ptr(a.gin, q), 0, q, rn);
         return a.shell_g[(long long)q * shell_total(a.shell) + shell_t];
     };
@@ -1145,6 +1159,92 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a, unsigned s
         }
     }
     fused_node<L, ST, CT, SP, DIRECT, MODE, SHELL, ROWS, false>(a, 0u);
+}
+
+// One launch per step for a 2D domain periodic on every side (plain
+// storage).  Each BX x BY block sums phi^{n+1} (k_phase's pull sum, same
+// loads and order) over its tile and a one-node halo into LDS.  Halo cells
+// in the ghost ring take what the ghost job of k_phase gives them: phi^n at
+// the wrapped interior node.  The block writes its tile and the ring cells
+// of its halo to the phi array (cells shared by several halos receive the
+// same value from each), then collides its tile with the stencil read from
+// LDS.  Interior halo cells are summed again by every block whose halo
+// holds them, with the same result, so the launch equals k_phase followed
+// by k_fused.
+template <class L, class ST, class CT, class SP>
+__global__ __launch_bounds__(BX* BY) void k_step_tile(StepArgs<ST, CT> a, const CT* phi_old) {
+    static_assert(L::D == 2 && !SP::uses_ref, "the tiled step serves 2D plain storage");
+    constexpr int TX = BX + 2, TY = BY + 2, HALO = 2 * TX + 2 * BY;
+    static_assert(HALO <= BX * BY, "one halo cell per thread at most");
+    __shared__ CT tile[TX * TY];
+    const DevGrid& g = a.g;
+    const int x0 = (int)(blockIdx.x * BX) + g.dmin[0] - 1, y0 = (int)(blockIdx.y * BY) + g.dmin[1] - 1;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    // cell 0: this thread's own node; cell 1: halo cell `tid` (top row, bottom
+    // row, then the left and right columns).  Both sums load before either
+    // result is stored.
+    int hx[2], hy[2];
+    hx[0] = threadIdx.x + 1;
+    hy[0] = threadIdx.y + 1;
+    if (tid < TX) {
+        hx[1] = tid;
+        hy[1] = 0;
+    } else if (tid < 2 * TX) {
+        hx[1] = tid - TX;
+        hy[1] = TY - 1;
+    } else {
+        const int k = tid - 2 * TX;
+        hx[1] = (k & 1) ? TX - 1 : 0;
+        hy[1] = 1 + (k >> 1);
+    }
+    CT v[2];
+    long long nn[2];
+    bool use[2], ring[2];
+    // the pulled f and g of this thread's node (cell 0), which its collision uses
+    CT pf[L::Q], pg[L::Q];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int c[3] = {x0 + hx[e], y0 + hy[e], g.dmin[2]};
+        // beyond the ring of a partial tile: no node the tile's stencils read
+        use[e] = (e == 0 || tid < HALO) && c[0] <= g.dmax[0] + 1 && c[1] <= g.dmax[1] + 1;
+        ring[e] = !g.interior(c, 2);
+        nn[e] = g.idx(c[0], c[1], c[2]);
+        v[e] = CT(0);
+        if (!use[e]) continue;
+        if (!ring[e]) {
+            const PullOffsets po = pull_offsets<L>(g, c);
+            const CT r = gref<SP>(a.phi_ref, nn[e]);
+            if (e == 0) {
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q) {
+                    const long long src = (long long)q * g.nodes + pull_src<L>(nn[0], po, q);
+                    pg[q] = SP::ld(a.gin, src, q, r);
+                    pf[q] = SP::ld(a.fin, src, q, CT(0));
+                }
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q) v[0] += pg[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q)
+                    v[1] += SP::ld(a.gin, (long long)q * g.nodes + pull_src<L>(nn[1], po, q), q, r);
+            }
+        } else {
+            int s[2];
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax)
+                s[ax] = c[ax] < g.dmin[ax] ? g.dmax[ax] : (c[ax] > g.dmax[ax] ? g.dmin[ax] : c[ax]);
+            v[e] = phi_old[g.idx(s[0], s[1], c[2])];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        if (!use[e]) continue;
+        tile[hy[e] * TX + hx[e]] = v[e];
+        // the owner writes an interior node; every halo holding a ring cell writes it
+        if (e == 0 || ring[e]) a.phi[nn[e]] = v[e];
+    }
+    __syncthreads();
+    fused_node<L, ST, CT, SP, false, MODE_FULL, false, false, false, true>(a, 0u, tile, pf, pg);
 }
 
 // ------------------------------------------------------------ node pairs

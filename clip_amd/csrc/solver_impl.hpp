@@ -409,6 +409,21 @@ struct SolverImpl : Impl {
         launch_check("k_shell_pull");
     }
 
+    // A 2D step periodic on every side with plain storage runs as one launch
+    // (k_step_tile: phase sum, ghost ring and collision per block).
+    bool tile_step() const {
+        if constexpr (L::D != 2 || SP::uses_ref) return false;
+        else return bc.both_periodic[0] && bc.both_periodic[1] && !bc.is_velocity;
+    }
+    void launch_step_tile(const StepArgs<ST, CT>& a, const CT* phi_old) {
+        if constexpr (L::D == 2 && !SP::uses_ref) {
+            const dim3 grid((unsigned)((g.dmax[0] - g.dmin[0] + BX) / BX),
+                            (unsigned)((g.dmax[1] - g.dmin[1] + BY) / BY), 1);
+            k_step_tile<L, ST, CT, SP><<<grid, dim3(BX, BY), 0, stream>>>(a, phi_old);
+            launch_check("k_step_tile");
+        }
+    }
+
     // Without inlet sides the ghost ring is one k_ghost_resolve pass, which
     // can ride on the phase launch (launch_phase_pull); total = 0 otherwise.
     GhostJob<CT> ghost_job(const CT* phi_old, CT* phi_new) const {
@@ -533,13 +548,18 @@ struct SolverImpl : Impl {
         StepArgs<ST, CT> a = args(cur, ucur, pcur, phi_new, phi_old);
         const bool tm = timing_on && !capturing;
         if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
+        const bool tiled = tile_step();
         // the ghost ring: with the phase launch when it is one pass
-        const GhostJob<CT> gj = ghost_job(phi_old, phi_new);
-        if (!gj.total) ghost_refresh(phi_old, phi_new);
+        GhostJob<CT> gj{};
+        if (!tiled) {
+            gj = ghost_job(phi_old, phi_new);
+            if (!gj.total) ghost_refresh(phi_old, phi_new);
+        }
         if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
-        launch_phase_pull(a, gj);
+        if (!tiled) launch_phase_pull(a, gj);
         if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
-        launch_fused_pull<MODE_FULL>(a);
+        if (tiled) launch_step_tile(a, phi_old);
+        else launch_fused_pull<MODE_FULL>(a);
         if (tm) {
             HIP_CHECK(hipEventRecord(ev[3], stream));
             HIP_CHECK(hipEventSynchronize(ev[3]));
@@ -1191,6 +1211,10 @@ struct SolverImpl : Impl {
                 s_.prev_g_in[s] = fc ? st_pg(a) + poff[s] : nullptr;
                 s_.prev_f_out[s] = fc ? st_pf(b) + poff[s] : nullptr;
                 s_.prev_g_out[s] = fc ? st_pg(b) + poff[s] : nullptr;
+            }
+            if (tile_step()) {
+                launch_step_tile(s_, st_phi(a));
+                return;
             }
             const GhostJob<CT> gj = ghost_job(st_phi(a), st_phi(b));
             if (!gj.total) ghost_refresh(st_phi(a), st_phi(b));

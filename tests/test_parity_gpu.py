@@ -175,6 +175,44 @@ def test_whole_row_blocks(name, text, steps):
         assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL, f
 
 
+# A 2D domain periodic on every side steps as one launch (each 64 x 4 block
+# sums phi^{n+1} over its tile and a one-node halo into LDS, writes the ghost
+# ring cells of its halo, then collides).  These shapes leave partial tiles on
+# either axis or both, fill exactly one tile, or are the narrowest interior
+# accepted.  mu and grad phi at the edge nodes read the ghost ring.
+TILED = {
+    "drop2d_67x37": cases.drop2d(n=67, ny=37, radius=10),
+    "drop2d_64x4": cases.drop2d(n=64, ny=4, radius=1.5),
+    "drop2d_130x9": cases.drop2d(n=130, ny=9, radius=3),
+    "drop2d_3x3": cases.drop2d(n=3, ny=3, radius=1.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TILED))
+def test_tiled_periodic_step_f64(name):
+    cfg, gpu, orc = run_pair(TILED[name], 80)
+    assert_fields(gpu, orc, ("phi", "u", "p", "rho", "mu", "dphi"))
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL, f
+
+
+# Not the 3 x 3 drop: its centre node has grad phi = 0 by symmetry while the
+# interface weight 1 - 4 (phi - 1/2)^2 is still ~0.8 there, so the normal is
+# rounding noise scaled to unit length.  The FP64 oracle built with FMA
+# contraction (a 7.6e-17 change after one step) departs from the canonical
+# oracle by 1.3e-2 after 10 steps on it; FP32 departs by the same order.
+@pytest.mark.parametrize("name", sorted(set(TILED) - {"drop2d_3x3"}))
+def test_tiled_periodic_step_f32(name):
+    cfg, gpu, orc = run_pair(TILED[name], 80, precision="f32")
+    e_phi = rel_l2(gpu.interior("phi"), orc.interior("phi"))
+    ua, ub = gpu.interior("u"), orc.interior("u")
+    du = np.linalg.norm((ua - ub).ravel())
+    print(name, f"fp32 rel-L2 phi {e_phi:.2e}, |du| {du:.2e} of |u| {np.linalg.norm(ub):.2e}", flush=True)
+    assert e_phi <= 2e-4
+    # u relative to its norm, floored at 1e-5 (the 3 x 3 drop's currents are ~1e-6)
+    assert du <= 1e-2 * max(np.linalg.norm(ub.ravel()), 1e-5)
+
+
 def test_sum_matches_interior_sum():
     cfg, gpu, orc = run_pair(cases.bubble3d(n=12, radius=3), 10)
     phi = gpu.interior("phi")
