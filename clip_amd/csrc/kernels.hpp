@@ -925,7 +925,8 @@ enum : int { MODE_FULL = 0, MODE_COLLIDE = 1 };
 // The node of this thread and its update.  CARRIED: a boundary-layer node
 // taken by the extra z-slices (from shell_first on) of a bulk launch.  TILE:
 // the phi stencil reads `tile`, the block's phi^{n+1} with a one-node halo in
-// LDS, rows of blockDim.x + 2 (k_step_tile), instead of the phi array, and
+// LDS, BX + 2 values a row (k_step_tile, BX x BY blocks), instead of the phi
+// array, and
 // the pulled f and g are the caller's registers pre_f / pre_g.
 template <class L, class ST, class CT, class SP, bool DIRECT, int MODE, bool SHELL, bool ROWS, bool CARRIED,
           bool TILE = false>
@@ -957,7 +958,7 @@ __device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned s
     if constexpr (TILE) {
         static_assert(L::D == 2 && !SP::uses_ref, "the LDS tile serves the 2D plain-storage step");
         DevGrid tg = g;
-        tg.sy = blockDim.x + 2;
+        tg.sy = BX + 2;
         phi_stencil<L, CT>(tile, tg, (long long)(threadIdx.y + 1) * tg.sy + threadIdx.x + 1, c0, dc, lap, nullptr);
     } else {
         phi_stencil<L, CT>(a.phi, g, n, c0, dc, lap, SP::uses_ref ? nbq : nullptr);
