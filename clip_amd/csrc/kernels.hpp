@@ -1169,9 +1169,11 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a, unsigned s
 // the wrapped interior node.  The block writes its tile and the ring cells
 // of its halo to the phi array (cells shared by several halos receive the
 // same value from each), then collides its tile with the stencil read from
-// LDS.  Interior halo cells are summed again by every block whose halo
-// holds them, with the same result, so the launch equals k_phase followed
-// by k_fused.
+// LDS and, for each node, the pulled f and g that its own sum loaded.
+// Interior halo cells are summed again by every block whose halo holds them,
+// with the same result, so the launch equals k_phase followed by k_fused.
+// Reads: the pull buffers and the old phi; writes: the new phi and the other
+// population buffers, so blocks never read what another block writes.
 template <class L, class ST, class CT, class SP>
 __global__ __launch_bounds__(BX* BY) void k_step_tile(StepArgs<ST, CT> a, const CT* phi_old) {
     static_assert(L::D == 2 && !SP::uses_ref, "the tiled step serves 2D plain storage");
