@@ -942,6 +942,8 @@ __device__ __forceinline__ void fused_node(const StepArgs<ST, CT>& a, unsigned s
             ((long long)(blockIdx.z - shell_first) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         shell_t = block * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
         if (!shell_node_at<L>(a.g, a.shell, shell_t, c)) return;
+    } else if constexpr (TILE) {
+        if (!node_of_thread<L>(a.g, a.bc, 0, c)) return;  // the tile spans the whole interior
     } else {
         if (!select_node<L, DIRECT, SHELL, ROWS>(a.g, a.bc, a.shell, c)) return;
         if (SHELL) shell_t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1162,94 +1164,6 @@ __global__ __launch_bounds__(BX* BY) void k_fused(StepArgs<ST, CT> a, unsigned s
     fused_node<L, ST, CT, SP, DIRECT, MODE, SHELL, ROWS, false>(a, 0u);
 }
 
-// One launch per step for a 2D domain periodic on every side (plain
-// storage).  Each BX x BY block sums phi^{n+1} (k_phase's pull sum, same
-// loads and order) over its tile and a one-node halo into LDS.  Halo cells
-// in the ghost ring take what the ghost job of k_phase gives them: phi^n at
-// the wrapped interior node.  The block writes its tile and the ring cells
-// of its halo to the phi array (cells shared by several halos receive the
-// same value from each), then collides its tile with the stencil read from
-// LDS and, for each node, the pulled f and g that its own sum loaded.
-// Interior halo cells are summed again by every block whose halo holds them,
-// with the same result, so the launch equals k_phase followed by k_fused.
-// Reads: the pull buffers and the old phi; writes: the new phi and the other
-// population buffers, so blocks never read what another block writes.
-template <class L, class ST, class CT, class SP>
-__global__ __launch_bounds__(BX* BY) void k_step_tile(StepArgs<ST, CT> a, const CT* phi_old) {
-    static_assert(L::D == 2 && !SP::uses_ref, "the tiled step serves 2D plain storage");
-    constexpr int TX = BX + 2, TY = BY + 2, HALO = 2 * TX + 2 * BY;
-    static_assert(HALO <= BX * BY, "one halo cell per thread at most");
-    __shared__ CT tile[TX * TY];
-    const DevGrid& g = a.g;
-    const int x0 = (int)(blockIdx.x * BX) + g.dmin[0] - 1, y0 = (int)(blockIdx.y * BY) + g.dmin[1] - 1;
-    const int tid = threadIdx.y * BX + threadIdx.x;
-    // cell 0: this thread's own node; cell 1: halo cell `tid` (top row, bottom
-    // row, then the left and right columns).  Both sums load before either
-    // result is stored.
-    int hx[2], hy[2];
-    hx[0] = threadIdx.x + 1;
-    hy[0] = threadIdx.y + 1;
-    if (tid < TX) {
-        hx[1] = tid;
-        hy[1] = 0;
-    } else if (tid < 2 * TX) {
-        hx[1] = tid - TX;
-        hy[1] = TY - 1;
-    } else {
-        const int k = tid - 2 * TX;
-        hx[1] = (k & 1) ? TX - 1 : 0;
-        hy[1] = 1 + (k >> 1);
-    }
-    CT v[2];
-    long long nn[2];
-    bool use[2], ring[2];
-    // the pulled f and g of this thread's node (cell 0), which its collision uses
-    CT pf[L::Q], pg[L::Q];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int c[3] = {x0 + hx[e], y0 + hy[e], g.dmin[2]};
-        // beyond the ring of a partial tile: no node the tile's stencils read
-        use[e] = (e == 0 || tid < HALO) && c[0] <= g.dmax[0] + 1 && c[1] <= g.dmax[1] + 1;
-        ring[e] = !g.interior(c, 2);
-        nn[e] = g.idx(c[0], c[1], c[2]);
-        v[e] = CT(0);
-        if (!use[e]) continue;
-        if (!ring[e]) {
-            const PullOffsets po = pull_offsets<L>(g, c);
-            const CT r = gref<SP>(a.phi_ref, nn[e]);
-            if (e == 0) {
-#pragma unroll
-                for (int q = 0; q < L::Q; ++q) {
-                    const long long src = (long long)q * g.nodes + pull_src<L>(nn[0], po, q);
-                    pg[q] = SP::ld(a.gin, src, q, r);
-                    pf[q] = SP::ld(a.fin, src, q, CT(0));
-                }
-#pragma unroll
-                for (int q = 0; q < L::Q; ++q) v[0] += pg[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < L::Q; ++q)
-                    v[1] += SP::ld(a.gin, (long long)q * g.nodes + pull_src<L>(nn[1], po, q), q, r);
-            }
-        } else {
-            int s[2];
-#pragma unroll
-            for (int ax = 0; ax < 2; ++ax)
-                s[ax] = c[ax] < g.dmin[ax] ? g.dmax[ax] : (c[ax] > g.dmax[ax] ? g.dmin[ax] : c[ax]);
-            v[e] = phi_old[g.idx(s[0], s[1], c[2])];
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        if (!use[e]) continue;
-        tile[hy[e] * TX + hx[e]] = v[e];
-        // the owner writes an interior node; every halo holding a ring cell writes it
-        if (e == 0 || ring[e]) a.phi[nn[e]] = v[e];
-    }
-    __syncthreads();
-    fused_node<L, ST, CT, SP, false, MODE_FULL, false, false, false, true>(a, 0u, tile, pf, pg);
-}
-
 // ------------------------------------------------------------ node pairs
 // 16-bit storage moves one population of one node per lane and access, and
 // a 19-plane pull at that width is bound by the access count rather than by
@@ -1399,12 +1313,14 @@ __device__ inline CT ghost_source(const CT* oldp, const CT* newp, const DevGrid&
 // (Solver.cu:927-978) of phi: ghost <- phi at the cell with every periodic
 // axis wrapped and every mirror axis clamped.  The source cell is never
 // written by the same launch.
-template <class CT>
-__device__ inline void ghost_resolve_cell(const CT* oldp, CT* newp, const DevGrid& g, const DevBC& bc, int D,
-                                          long long t, int do_periodic, int do_mirror, int ghost_from_new) {
-    int c[3];
-    if (!ghost_cell_of_thread(g, D, t, c)) return;
-    int s[3] = {c[0], c[1], c[2]};
+// The cell whose phi ghost cell c copies: every periodic axis wrapped and
+// every mirror axis clamped; false when neither applies (the cell keeps its
+// value).
+__device__ inline bool ghost_source_cell(const DevGrid& g, const DevBC& bc, int D, const int c[3], int do_periodic,
+                                         int do_mirror, int s[3]) {
+    s[0] = c[0];
+    s[1] = c[1];
+    s[2] = c[2];
     bool changed = false;
     for (int a = 0; a < D; ++a) {
         const bool lo = c[a] < g.dmin[a], hi = c[a] > g.dmax[a];
@@ -1422,13 +1338,140 @@ __device__ inline void ghost_resolve_cell(const CT* oldp, CT* newp, const DevGri
             changed = true;
         }
     }
-    if (changed) newp[g.idx(c[0], c[1], c[2])] = ghost_source(oldp, newp, g, D, s, ghost_from_new);
+    return changed;
+}
+template <class CT>
+__device__ inline void ghost_resolve_cell(const CT* oldp, CT* newp, const DevGrid& g, const DevBC& bc, int D,
+                                          long long t, int do_periodic, int do_mirror, int ghost_from_new) {
+    int c[3];
+    if (!ghost_cell_of_thread(g, D, t, c)) return;
+    int s[3];
+    if (ghost_source_cell(g, bc, D, c, do_periodic, do_mirror, s))
+        newp[g.idx(c[0], c[1], c[2])] = ghost_source(oldp, newp, g, D, s, ghost_from_new);
 }
 template <class CT>
 __global__ void k_ghost_resolve(const CT* oldp, CT* newp, DevGrid g, DevBC bc, int D, long long total,
                                 int do_periodic, int do_mirror, int ghost_from_new) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < total) ghost_resolve_cell(oldp, newp, g, bc, D, t, do_periodic, do_mirror, ghost_from_new);
+}
+
+// One launch per step for a 2D domain periodic along x (plain storage, no
+// inlet or free-convect side).  Each BX x BY block sums phi^{n+1} (k_phase's
+// pull sum, same loads and order) over its tile and a one-node halo into LDS;
+// halo cells in the ghost ring take what the ghost job gives them (phi^n at
+// the wrapped or mirrored node, ghost_source_cell).  The block writes its
+// tile's bulk nodes and the ring cells of its halo to the phi array (cells
+// shared by several halos receive the same value from each), then collides
+// its tile with the stencil read from LDS and, for each node, the pulled f
+// and g that its own sum loaded.  LAYER: y is not periodic on both sides, so
+// the rows next to those sides are the boundary layer, whose phi^{n+1} and
+// post-streaming f and g k_shell_phi / k_shell_pull have already produced
+// (phi array, layer scratch); those rows collide here too, from that scratch.
+// Interior halo cells are summed again by every block whose halo holds them,
+// with the same result, so the launch equals k_phase followed by k_fused.
+// Reads: the pull buffers, the old phi and the layer scratch; writes: the new
+// phi (bulk and ring) and the other population buffers, so blocks never read
+// what another block writes.
+template <class L, class ST, class CT, class SP, bool LAYER>
+__global__ __launch_bounds__(BX* BY) void k_step_tile(StepArgs<ST, CT> a, const CT* phi_old) {
+    static_assert(L::D == 2 && !SP::uses_ref, "the tiled step serves 2D plain storage");
+    constexpr int TX = BX + 2, TY = BY + 2, HALO = 2 * TX + 2 * BY;
+    static_assert(HALO <= BX * BY, "one halo cell per thread at most");
+    __shared__ CT tile[TX * TY];
+    const DevGrid& g = a.g;
+    const int x0 = (int)(blockIdx.x * BX) + g.dmin[0] - 1, y0 = (int)(blockIdx.y * BY) + g.dmin[1] - 1;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    // cell 0: this thread's own node; cell 1: halo cell `tid` (top row, bottom
+    // row, then the left and right columns).  Both load before either result
+    // is stored.
+    int hx[2], hy[2];
+    hx[0] = threadIdx.x + 1;
+    hy[0] = threadIdx.y + 1;
+    if (tid < TX) {
+        hx[1] = tid;
+        hy[1] = 0;
+    } else if (tid < 2 * TX) {
+        hx[1] = tid - TX;
+        hy[1] = TY - 1;
+    } else {
+        const int k = tid - 2 * TX;
+        hx[1] = (k & 1) ? TX - 1 : 0;
+        hy[1] = 1 + (k >> 1);
+    }
+    CT v[2];
+    long long nn[2];
+    bool use[2], store[2];
+    // the pulled f and g of this thread's node (cell 0), which its collision uses
+    CT pf[L::Q], pg[L::Q];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int c[3] = {x0 + hx[e], y0 + hy[e], g.dmin[2]};
+        // beyond the ring of a partial tile: no node the tile's stencils read
+        use[e] = (e == 0 || tid < HALO) && c[0] <= g.dmax[0] + 1 && c[1] <= g.dmax[1] + 1;
+        nn[e] = g.idx(c[0], c[1], c[2]);
+        v[e] = CT(0);
+        store[e] = false;
+        if (!use[e]) continue;
+        const bool layer = LAYER && (c[1] == g.dmin[1] || c[1] == g.dmax[1]);
+        if (!g.interior(c, 2)) {
+            int s[3] = {c[0], c[1], c[2]};
+            bool copied = true;
+            if constexpr (LAYER) {
+                copied = ghost_source_cell(g, a.bc, 2, c, 1, 1, s);
+            } else {
+                // every side periodic: the copy is the wrap on both axes
+#pragma unroll
+                for (int ax = 0; ax < 2; ++ax)
+                    s[ax] = c[ax] < g.dmin[ax] ? g.dmax[ax] : (c[ax] > g.dmax[ax] ? g.dmin[ax] : c[ax]);
+            }
+            if (copied) {
+                v[e] = phi_old[g.idx(s[0], s[1], s[2])];
+                store[e] = true;
+            } else {
+                v[e] = a.phi[nn[e]];  // a ring cell no copy reaches keeps its value
+            }
+        } else if (layer) {
+            v[e] = a.phi[nn[e]];
+            if (e == 0) {
+                // the layer's scratch: the lower row, then the upper one
+                const long long S = shell_total(a.shell), n0 = g.dmax[0] - g.dmin[0] + 1;
+                const long long t = (c[1] == g.dmax[1] ? n0 : 0) + (c[0] - g.dmin[0]);
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q) {
+                    pf[q] = a.shell_f[(long long)q * S + t];
+                    pg[q] = a.shell_g[(long long)q * S + t];
+                }
+            }
+        } else {
+            const PullOffsets po = pull_offsets<L>(g, c);
+            const CT r = gref<SP>(a.phi_ref, nn[e]);
+            if (e == 0) {
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q) {
+                    const long long src = (long long)q * g.nodes + pull_src<L>(nn[0], po, q);
+                    pg[q] = SP::ld(a.gin, src, q, r);
+                    pf[q] = SP::ld(a.fin, src, q, CT(0));
+                }
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q) v[0] += pg[q];
+                store[0] = true;
+            } else {
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q)
+                    v[1] += SP::ld(a.gin, (long long)q * g.nodes + pull_src<L>(nn[1], po, q), q, r);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        if (!use[e]) continue;
+        tile[hy[e] * TX + hx[e]] = v[e];
+        // the owner writes a bulk node; every halo holding a copied ring cell writes it
+        if (store[e]) a.phi[nn[e]] = v[e];
+    }
+    __syncthreads();
+    fused_node<L, ST, CT, SP, false, MODE_FULL, false, false, false, true>(a, 0u, tile, pf, pg);
 }
 
 // Inlet side, outside the disc: phi ghost <- boundary node (Solver.cu:621-633).

@@ -409,17 +409,34 @@ struct SolverImpl : Impl {
         launch_check("k_shell_pull");
     }
 
-    // A 2D step periodic on every side with plain storage runs as one launch
-    // (k_step_tile: phase sum, ghost ring and collision per block).
-    bool tile_step() const {
-        if constexpr (L::D != 2 || SP::uses_ref) return false;
-        else return bc.both_periodic[0] && bc.both_periodic[1] && !bc.is_velocity;
+    // 2D steps with plain storage, x periodic on both sides and no inlet or
+    // free-convect side run the phase sum, the ghost ring and the collision
+    // as one k_step_tile launch: 1 = every side periodic; 2 = y bounded, its
+    // boundary layer (the two rows next to the y sides) pulled first by
+    // k_shell_pull / k_shell_phi; 0 = the separate launches.
+    int tile_mode() const {
+        if constexpr (L::D != 2 || SP::uses_ref) {
+            return 0;
+        } else {
+            if (bc.is_velocity || bc.is_fc || !bc.both_periodic[0]) return 0;
+            if (bc.both_periodic[1]) return 1;
+            const ShellDesc sd = make_shell<L>(g, bc);
+            return (sd.na == 1 && sd.axes[0] == 1) ? 2 : 0;
+        }
     }
-    void launch_step_tile(const StepArgs<ST, CT>& a, const CT* phi_old) {
+    bool tile_step() const { return tile_mode() == 1; }
+    void launch_step_tile(const StepArgs<ST, CT>& a, const CT* phi_old, int mode) {
         if constexpr (L::D == 2 && !SP::uses_ref) {
             const dim3 grid((unsigned)((g.dmax[0] - g.dmin[0] + BX) / BX),
                             (unsigned)((g.dmax[1] - g.dmin[1] + BY) / BY), 1);
-            k_step_tile<L, ST, CT, SP><<<grid, dim3(BX, BY), 0, stream>>>(a, phi_old);
+            if (mode == 2) {
+                launch_shell_pull(a);
+                k_shell_phi<L, ST, CT><<<shell_blocks(), BX * BY, 0, stream>>>(a);
+                launch_check("k_shell_phi");
+                k_step_tile<L, ST, CT, SP, true><<<grid, dim3(BX, BY), 0, stream>>>(a, phi_old);
+            } else {
+                k_step_tile<L, ST, CT, SP, false><<<grid, dim3(BX, BY), 0, stream>>>(a, phi_old);
+            }
             launch_check("k_step_tile");
         }
     }
@@ -548,7 +565,7 @@ struct SolverImpl : Impl {
         StepArgs<ST, CT> a = args(cur, ucur, pcur, phi_new, phi_old);
         const bool tm = timing_on && !capturing;
         if (tm) HIP_CHECK(hipEventRecord(ev[0], stream));
-        const bool tiled = tile_step();
+        const int tiled = tile_mode();
         // the ghost ring: with the phase launch when it is one pass
         GhostJob<CT> gj{};
         if (!tiled) {
@@ -558,7 +575,7 @@ struct SolverImpl : Impl {
         if (tm) HIP_CHECK(hipEventRecord(ev[1], stream));
         if (!tiled) launch_phase_pull(a, gj);
         if (tm) HIP_CHECK(hipEventRecord(ev[2], stream));
-        if (tiled) launch_step_tile(a, phi_old);
+        if (tiled) launch_step_tile(a, phi_old, tiled);
         else launch_fused_pull<MODE_FULL>(a);
         if (tm) {
             HIP_CHECK(hipEventRecord(ev[3], stream));
@@ -1213,7 +1230,7 @@ struct SolverImpl : Impl {
                 s_.prev_g_out[s] = fc ? st_pg(b) + poff[s] : nullptr;
             }
             if (tile_step()) {
-                launch_step_tile(s_, st_phi(a));
+                launch_step_tile(s_, st_phi(a), 1);
                 return;
             }
             const GhostJob<CT> gj = ghost_job(st_phi(a), st_phi(b));

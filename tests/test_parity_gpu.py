@@ -213,6 +213,38 @@ def test_tiled_periodic_step_f32(name):
     assert du <= 1e-2 * max(np.linalg.norm(ub.ravel()), 1e-5)
 
 
+# x periodic, y bounded: the same single launch collides the boundary-layer
+# rows from the layer pull's scratch and mirrors or keeps the y ghost cells
+# as the ghost job does.  Row counts off the 4-row tile, partial x tiles, and
+# wall, do-nothing, Neumann and slip-wall y sides.
+TILED_LAYER = {
+    "rti2d_30x37": cases.rti2d(30, 37),
+    "box2d_67x37_wall_donothing": cases.box2d([("x-", "periodic"), ("x+", "periodic"), ("y-", "wall"),
+                                               ("y+", "do nothing")], n=67, ny=37),
+    "box2d_64x5_neumann_slip": cases.box2d([("x-", "periodic"), ("x+", "periodic"), ("y-", "neumann"),
+                                            ("y+", "slip wall")], n=64, ny=5, radius=1.5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TILED_LAYER))
+def test_tiled_layer_step_f64(name):
+    cfg, gpu, orc = run_pair(TILED_LAYER[name], 80)
+    assert_fields(gpu, orc, ("phi", "u", "p", "rho", "mu", "dphi"))
+    for f in ("f_post", "g_post"):
+        assert rel_l2(gpu.interior(f), orc.interior(f)) <= TOL, f
+
+
+@pytest.mark.parametrize("name", sorted(TILED_LAYER))
+def test_tiled_layer_step_f32(name):
+    cfg, gpu, orc = run_pair(TILED_LAYER[name], 80, precision="f32")
+    e_phi = rel_l2(gpu.interior("phi"), orc.interior("phi"))
+    ua, ub = gpu.interior("u"), orc.interior("u")
+    du = np.linalg.norm((ua - ub).ravel())
+    print(name, f"fp32 rel-L2 phi {e_phi:.2e}, |du| {du:.2e} of |u| {np.linalg.norm(ub):.2e}", flush=True)
+    assert e_phi <= 2e-4
+    assert du <= 1e-2 * max(np.linalg.norm(ub.ravel()), 1e-5)
+
+
 def test_sum_matches_interior_sum():
     cfg, gpu, orc = run_pair(cases.bubble3d(n=12, radius=3), 10)
     phi = gpu.interior("phi")
